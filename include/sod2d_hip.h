@@ -1,0 +1,201 @@
+/*
+ * sod2d_hip.h -- C ABI of libsod2d_hip.so, the MI355X (gfx950) implementation
+ * of SOD2D's explicit compressible spectral-element RHS + RK4 step
+ * (HEX64, p = 3, FP64, one partition on one GPU).
+ *
+ * Every hot-path entry point replaces one Fortran routine of the reference
+ * and keeps its argument list, order and meaning (reference file:line cited
+ * per declaration; paths relative to the reference root).  Conventions:
+ *   - arrays are the reference's column-major arrays with the same extents;
+ *     index arrays hold the reference's 1-based values;
+ *   - array arguments are device pointers, or host pointers registered with
+ *     sod_enter_data() (the OpenACC "enter data"/"present" model of the
+ *     reference, time_integ.f90:37-68, CFDSolverBase.f90:827-842);
+ *   - scalars are passed by value (the Fortran interface declares VALUE);
+ *     Fortran LOGICAL arguments are int (0 = .false.);
+ *   - compile-time constants porder=3, nnode=ngaus=64, ndime=3 are not
+ *     arguments (mod_constants.f90:12-20); sod_init() checks porder;
+ *   - intent(out) arrays are fully overwritten, as in the reference;
+ *   - return 0 on success, a negative SOD_ERR_* code otherwise, with the
+ *     message in sod_last_error() (the reference has no error returns: its
+ *     routines stop the program; the Fortran shim turns a non-zero return
+ *     into ERROR STOP);
+ *   - calls are synchronous on return unless sod_set_sync(0) was called;
+ *     work is queued on the stream given to sod_set_stream() (default: the
+ *     null stream of the current device).
+ */
+#ifndef SOD2D_HIP_H
+#define SOD2D_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SOD_OK 0
+#define SOD_ERR_ARG (-1)         /* invalid argument or unsupported value */
+#define SOD_ERR_HIP (-2)         /* HIP runtime error */
+#define SOD_ERR_NOT_PRESENT (-3) /* host pointer not registered with sod_enter_data */
+#define SOD_ERR_PORDER (-4)      /* porder != 3 */
+#define SOD_ERR_UNSUPPORTED (-5) /* feature outside the periodic explicit path */
+#define SOD_ERR_NOT_INIT (-6)
+
+/* Module globals of mod_numerical_params.f90 read by the path. */
+typedef struct {
+    double ce;                  /* mod_numerical_params.f90:24, default 0.1 */
+    double cmax;                /* :25, default 0.5 */
+    double cglob;               /* :26, default 1.0 */
+    double c_rho;               /* :27, default 1.0 */
+    double c_ener;              /* :28, default 1.0 */
+    double Rgas_inf;            /* nscbc_Rgas_inf, :45 (the case sets it, TGVSolver.f90:104) */
+    int flag_normalise_entropy; /* :23, default 1 */
+    int flag_total_enthalpy;    /* :20, default 0 */
+    int flag_rk_order;          /* :8, only 4 is supported */
+    int flag_les;               /* :13, only 0 is supported */
+    int flag_real_diff;         /* :93, default 1 */
+    int flag_diff_suth;         /* :94, default 1 */
+} sod_numerical_params;
+
+/* ----------------------------------------------------------- runtime ---- */
+
+/* No counterpart (the reference initialises OpenACC implicitly).  Selects
+ * the current HIP device and checks porder == 3 (mod_hdf5.f90:2781-2784 stops
+ * on a porder mismatch). */
+int sod_init(int porder);
+int sod_finalize(void);
+const char *sod_last_error(void);
+int sod_set_stream(void *hip_stream);
+int sod_set_sync(int sync_on_return);
+int sod_synchronize(void);
+/* mod_numerical_params.f90:1-100 module variables. */
+int sod_set_numerical_params(const sod_numerical_params *p);
+int sod_get_numerical_params(sod_numerical_params *p);
+void sod_default_numerical_params(sod_numerical_params *p);
+
+/* OpenACC data directives of the reference: enter data create/copyin
+ * (time_integ.f90:37-68), update device/host (CFDSolverBase.f90:1182-1226),
+ * exit data delete (time_integ.f90:97-135).  `host` may be any address inside
+ * a registered block for update/deviceptr. */
+int sod_enter_data(const void *host, size_t bytes, int copyin);
+int sod_exit_data(const void *host, int copyout);
+int sod_update_device(const void *host, size_t bytes);
+int sod_update_host(void *host, size_t bytes);
+void *sod_deviceptr(const void *host);
+
+/* ------------------------------------------------- init-time tables ---- */
+
+/* GaussLobattoLegendre_hex (quadrature_rules.f90:51-78), hex_highorder /
+ * TripleTensorProduct (elem_hex.f90:22-43, mod_maths.f90:258-298),
+ * atoIJKInverse (mod_geom.f90:123-144) with a2ijk = gmsh2ijk
+ * (mod_meshConversorTool.f90:150-152, mod_ijk_indices.f90:61-114).
+ * Host-only: needs no GPU.  Outputs are host arrays:
+ * atoIJK(64), xgp(64,3), wgp(64), Ngp(64,64), dNgp(3,64,64),
+ * dlxigp_ip(64,3,4), invAtoIJK(4,4,4), gmshAtoI/J/K(64). */
+int sod_hex64_tables(int *atoIJK, double *xgp, double *wgp, double *Ngp, double *dNgp,
+                     double *dlxigp_ip, int *invAtoIJK, int *gmshAtoI, int *gmshAtoJ,
+                     int *gmshAtoK);
+
+/* elem_jacobian, jacobian_oper.f90:9-138 */
+int sod_elem_jacobian(int nelem, int npoin, const int *connec, const double *coord,
+                      const double *dNgp, const double *wgp, double *gpvol, double *He);
+/* lumped_mass_spectral, mass_matrix.f90:11-42 */
+int sod_lumped_mass_spectral(int nelem, int npoin, const int *connec, const double *gpvol,
+                             double *Ml);
+/* char_length over every element (CFDSolverBase.f90:791-794 calling
+ * mod_geom.f90:13-37) -> helem(nelem) */
+int sod_char_length(int nelem, int npoin, const int *connec, const double *coord,
+                    double *helem);
+/* char_length_spectral over every element (CFDSolverBase.f90:1378-1380
+ * calling mod_geom.f90:39-66) -> helem_l(nelem,64) */
+int sod_char_length_spectral(int nelem, int npoin, const int *connec, const double *coord,
+                             double *helem_l);
+
+/* --------------------------------------------------------- hot path ---- */
+
+/* full_convec_ijk, elem_convec.f90:211-224 (Ngp, dNgp and xgp are accepted
+ * and unused, as in the reference). */
+int sod_full_convec_ijk(int nelem, int npoin, const int *connec, const double *Ngp,
+                        const double *dNgp, const double *He, const double *gpvol,
+                        const double *dlxigp_ip, const double *xgp, const int *invAtoIJK,
+                        const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                        const double *u, const double *q, const double *rho, const double *pr,
+                        const double *E, double *Rmass, double *Rmom, double *Rener);
+/* full_convec_ijk_H, elem_convec.f90:20-33 */
+int sod_full_convec_ijk_H(int nelem, int npoin, const int *connec, const double *Ngp,
+                          const double *dNgp, const double *He, const double *gpvol,
+                          const double *dlxigp_ip, const double *xgp, const int *invAtoIJK,
+                          const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                          const double *u, const double *q, const double *rho, const double *pr,
+                          const double *E, double *Rmass, double *Rmom, double *Rener);
+/* generic_scalar_convec_ijk, elem_convec.f90:462-476 */
+int sod_generic_scalar_convec_ijk(int nelem, int npoin, const int *connec, const double *Ngp,
+                                  const double *dNgp, const double *He, const double *gpvol,
+                                  const double *dlxigp_ip, const double *xgp,
+                                  const int *invAtoIJK, const int *gmshAtoI,
+                                  const int *gmshAtoJ, const int *gmshAtoK, const double *q,
+                                  const double *eta, const double *u, double *Rconvec);
+/* full_diffusion_ijk, elem_diffu.f90:12-25 */
+int sod_full_diffusion_ijk(int nelem, int npoin, const int *connec, const double *Ngp,
+                           const double *He, const double *gpvol, const double *dlxigp_ip,
+                           const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+                           const int *gmshAtoK, double Cp, double Pr, const double *rho_n,
+                           const double *rho, const double *u, const double *Tem,
+                           const double *mu_fluid, const double *mu_e, const double *mu_sgs,
+                           const double *Ml, double *Rmass, double *Rmom, double *Rener);
+/* sutherland_viscosity, mod_fluid_viscosity.f90:21-41 */
+int sod_sutherland_viscosity(int npoin, const double *Tem, const double *mu_factor,
+                             double *mu_fluid);
+/* lumped_solver_scal / lumped_solver_vect, mod_solver.f90:23-58 */
+int sod_lumped_solver_scal(int npoin, int npoin_w, const int *lpoin_w, const double *Ml,
+                           double *R);
+int sod_lumped_solver_vect(int npoin, int npoin_w, const int *lpoin_w, const double *Ml,
+                           double *R);
+/* smart_visc_spectral, mod_entropy_viscosity.f90:14-27 (the dummy helem is
+ * helem_l(nelem,64), helem_k is helem(nelem), as at time_integ.f90:492). */
+int sod_smart_visc_spectral(int nelem, int npoin, int npoin_w, const int *connec,
+                            const int *lpoin_w, const double *Reta, const double *Ngp,
+                            const double *coord, const double *dNgp, const double *gpvol,
+                            const double *wgp, double gamma_gas, const double *rho,
+                            const double *u, const double *csound, const double *Tem,
+                            const double *eta, const double *helem, const double *helem_k,
+                            const double *Ml, double *mu_e, const int *invAtoIJK,
+                            const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                            double *mue_l);
+/* adapt_dt_cfl, mod_time_ops.f90:11-61.  `dt` is a HOST scalar (inout).  The
+ * optional arguments cfl_diff/mu_fluid/mu_sgs/rho are all given or all NULL
+ * (cfl_diff is passed by address for that reason). */
+int sod_adapt_dt_cfl(int nelem, int npoin, const int *connec, const double *helem,
+                     const double *u, const double *csound, double cfl_conv, double *dt,
+                     const double *cfl_diff, const double *mu_fluid, const double *mu_sgs,
+                     const double *rho);
+/* volAvg_EK, mod_analysis.f90:10-44.  `EK` is a HOST scalar (out). */
+int sod_volAvg_EK(int nelem, int npoin, const int *connec, const double *gpvol,
+                  const double *Ngp, double rho0, const double *rho, const double *u,
+                  double *EK);
+
+/* rk_4_main, time_integ.f90:137-191: one RK4 step of the periodic explicit
+ * path.  State arrays carry the reference's time levels (rho(npoin,3),
+ * u(npoin,3,3), q(npoin,3,3), pr(npoin,3), E(npoin,3), Tem(npoin,2),
+ * e_int(npoin,2), eta(npoin,3)) and level pos=2 is advanced in place.
+ * noBoundaries must be true, isWallModelOn false (or numBoundsWM 0) and
+ * flag_les 0; the boundary-only arguments (point2elem, lnbn_nodes, nboun)
+ * are accepted and unused, as the reference ignores them in this branch. */
+int sod_rk_4_main(int noBoundaries, int isWallModelOn, int nelem, int nboun, int npoin,
+                  int npoin_w, int numBoundsWM, const int *point2elem, const int *lnbn_nodes,
+                  const double *dlxigp_ip, const double *xgp, const int *atoIJK,
+                  const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+                  const int *gmshAtoK, int ppow, const int *connec, const double *Ngp,
+                  const double *dNgp, const double *coord, const double *wgp, const double *He,
+                  const double *Ml, const double *gpvol, double dt, const double *helem,
+                  const double *helem_l, double Rgas, double gamma_gas, double Cp, double Prt,
+                  double *rho, double *u, double *q, double *pr, double *E, double *Tem,
+                  double *csound, double *machno, double *e_int, double *eta, double *mu_e,
+                  double *mu_sgs, double *kres, double *etot, double *au, double *ax1,
+                  double *ax2, double *ax3, const int *lpoin_w, double *mu_fluid,
+                  const double *mu_factor, double *mue_l);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

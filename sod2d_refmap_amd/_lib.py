@@ -1,0 +1,103 @@
+"""ctypes binding of libsod2d_hip.so (the C ABI declared in include/sod2d_hip.h).
+
+The product path always runs through this library: if it is missing the
+import of any compute function raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsod2d_hip.so")
+
+c_int, c_double, c_size_t, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p
+P = c_void_p  # every array argument is passed as an address
+
+SOD_OK = 0
+
+
+class Sod2dError(RuntimeError):
+    """A non-zero return code of the C ABI, with sod_last_error()'s message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libsod2d_hip error {code}: {msg}")
+        self.code = code
+
+
+class NumericalParams(ctypes.Structure):
+    """mod_numerical_params.f90 module variables read by the path."""
+    _fields_ = [
+        ("ce", c_double), ("cmax", c_double), ("cglob", c_double), ("c_rho", c_double),
+        ("c_ener", c_double), ("Rgas_inf", c_double), ("flag_normalise_entropy", c_int),
+        ("flag_total_enthalpy", c_int), ("flag_rk_order", c_int), ("flag_les", c_int),
+        ("flag_real_diff", c_int), ("flag_diff_suth", c_int),
+    ]
+
+
+_SIGS = {
+    "sod_init": [c_int],
+    "sod_finalize": [],
+    "sod_set_stream": [c_void_p],
+    "sod_set_sync": [c_int],
+    "sod_synchronize": [],
+    "sod_set_numerical_params": [ctypes.POINTER(NumericalParams)],
+    "sod_get_numerical_params": [ctypes.POINTER(NumericalParams)],
+    "sod_enter_data": [P, c_size_t, c_int],
+    "sod_exit_data": [P, c_int],
+    "sod_update_device": [P, c_size_t],
+    "sod_update_host": [P, c_size_t],
+    "sod_hex64_tables": [P] * 10,
+    "sod_elem_jacobian": [c_int, c_int] + [P] * 6,
+    "sod_lumped_mass_spectral": [c_int, c_int, P, P, P],
+    "sod_char_length": [c_int, c_int, P, P, P],
+    "sod_char_length_spectral": [c_int, c_int, P, P, P],
+    "sod_full_convec_ijk": [c_int, c_int] + [P] * 19,
+    "sod_full_convec_ijk_H": [c_int, c_int] + [P] * 19,
+    "sod_generic_scalar_convec_ijk": [c_int, c_int] + [P] * 15,
+    "sod_full_diffusion_ijk": [c_int, c_int] + [P] * 9 + [c_double, c_double] + [P] * 11,
+    "sod_sutherland_viscosity": [c_int, P, P, P],
+    "sod_lumped_solver_scal": [c_int, c_int, P, P, P],
+    "sod_lumped_solver_vect": [c_int, c_int, P, P, P],
+    "sod_smart_visc_spectral": [c_int, c_int, c_int] + [P] * 8 + [c_double] + [P] * 14,
+    "sod_adapt_dt_cfl": [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P],
+    "sod_volAvg_EK": [c_int, c_int, P, P, P, c_double, P, P, P],
+    "sod_rk_4_main": ([c_int] * 7 + [P] * 9 + [c_int] + [P] * 8 + [c_double] + [P] * 2 +
+                      [c_double] * 4 + [P] * 22),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built: run `python sod2d_refmap_amd/build.py` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        _lib = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(_lib, name)
+            fn.argtypes = args
+            fn.restype = c_int
+        _lib.sod_last_error.restype = ctypes.c_char_p
+        _lib.sod_last_error.argtypes = []
+        _lib.sod_default_numerical_params.restype = None
+        _lib.sod_default_numerical_params.argtypes = [ctypes.POINTER(NumericalParams)]
+        _lib.sod_deviceptr.restype = c_void_p
+        _lib.sod_deviceptr.argtypes = [c_void_p]
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != SOD_OK:
+        raise Sod2dError(rc, lib().sod_last_error().decode(errors="replace"))
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))
+
+
+def exported_symbols() -> list[str]:
+    return sorted(list(_SIGS) + ["sod_last_error", "sod_default_numerical_params", "sod_deviceptr"])
