@@ -186,6 +186,8 @@ int rk4_step_reference_sequence(int nelem, int npoin, int npoin_w, const int* co
     for (int istep = 0; istep < 4; ++istep) {
         hipLaunchKernelGGL(k_stage_state, gN, b, 0, s, npoin, istep, dt, rho, E, q, W);
         hipLaunchKernelGGL(k_stage_eos, gW, b, 0, s, npoin, npoin_w, lpoin_w, gamma_gas, Rgas, W);
+        // generic_scalar_convec_ijk zeroes its output (elem_convec.f90:484-486)
+        SOD_HIP(hipMemsetAsync(W + F_RETA * np, 0, sizeof(double) * np, s));
         launch_scalar_convec(nelem, npoin, connec, He, gpvol, W + F_FETA0 * np, W + F_AUX_ETA * np,
                              W + F_AUX_U0 * np, W + F_RETA * np);
         launch_lumped(npoin, npoin_w, 1, lpoin_w, Ml, W + F_RETA * np);

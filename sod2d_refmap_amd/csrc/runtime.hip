@@ -90,10 +90,8 @@ int resolve(const void* p, const char* what, const void** out) {
         *out = p;
         return SOD_OK;
     }
-    if (attr.type == hipMemoryTypeHost && attr.devicePointer != nullptr) {
-        *out = attr.devicePointer;   // pinned host memory, mapped
-        return SOD_OK;
-    }
+    // host memory of any kind (pageable or pinned) must go through the
+    // present table, as OpenACC's "present" clause requires
     return fail(SOD_ERR_NOT_PRESENT,
                 "argument %s (%p) is host memory that is not present on the device; "
                 "register it with sod_enter_data()", what, p);
