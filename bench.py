@@ -1,0 +1,162 @@
+"""Benchmark: FP64 HEX64 explicit RK4 step of the compressible Taylor-Green
+vortex on one MI355X, reported as GDOF/s per RK4 stage.
+
+A step is rk_4_main (4 RHS stages + stage updates + entropy viscosity) plus
+the CFL time-step evaluation that the reference's time loop runs after it,
+on a periodic box of n^3 HEX64 elements (default n = 64: 262,144 elements,
+7,077,888 working nodes).  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line (see the fields below).  --gpus N with N > 1 exits: this
+build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def kernel_ms(name):
+    from sod2d_refmap_amd import _lib
+    tot, cnt = ctypes.c_double(0.0), ctypes.c_int(0)
+    _lib.call("sod_profile_get", name.encode(), ctypes.byref(tot), ctypes.byref(cnt))
+    return tot.value, cnt.value
+
+
+def cpu_baseline(seconds: float):
+    """The oracle (single-threaded C restatement of the reference routines)
+    stepping the reference's CPU case: TGV on 8^3 HEX64 elements."""
+    import numpy as np
+    import oracle
+    from sod2d_refmap_amd import cases, mesh
+    m = mesh.periodic_box(8)
+    gas = cases.GasParams()
+    s = cases.tgv_initial_state(m, gas)
+    t = oracle.hex64_tables()
+    gpvol, He = oracle.elem_jacobian(m.connec_orig, m.coord, t)
+    Ml = oracle.lumped_mass_spectral(m.connec, gpvol, m.npoin)
+    helem = oracle.char_length(m.connec_orig, m.coord)
+    helem_l = oracle.char_length_spectral(m.connec_orig, m.coord)
+    p = oracle.default_params(Rgas_inf=gas.Rgas)
+    zero = np.zeros((m.nelem, 64), order="F")
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        dt = oracle.adapt_dt_cfl(m.connec, helem, s["u"], s["csound"], 0.5, 0.5, s["mu_fluid"], zero, s["rho"])
+        oracle.rk_4_main(m.connec, m.lpoin_w, He, gpvol, t, Ml, helem, helem_l, dt, gas.Rgas, gas.gamma_gas,
+                         gas.Cp, gas.Prt, s["mu_factor"], s, p)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 200:
+            break
+    gdofs = 5.0 * m.npoin_w * 4 * steps / el / 1e9
+    return {"value": gdofs, "unit": "GDOF/s per RK4 stage", "cores": 1, "kind": "port",
+            "sample": f"oracle rk_4_main + adapt_dt_cfl, TGV 8^3 HEX64 ({m.npoin_w} nodes), {steps} steps "
+                      f"in {el:.1f} s on 1 host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64, help="elements per direction")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from sod2d_refmap_amd import _lib, cases, mesh
+    from sod2d_refmap_amd.solver import PeriodicSolver
+
+    n = args.n
+    t0 = time.perf_counter()
+    m = mesh.periodic_box(n)
+    gas = cases.GasParams()
+    s = cases.tgv_initial_state(m, gas)
+    log(f"mesh {n}^3: nelem={m.nelem} npoin_w={m.npoin_w} ({time.perf_counter() - t0:.1f}s)")
+    sol = PeriodicSolver(m, gas, s, cfl_conv=0.5, cfl_diff=0.5)
+    del s
+    dt = sol.adapt_dt()
+    log(f"setup done, dt0={dt:.4e}")
+    for _ in range(args.warmup):
+        sol.rk_4_main(sol.dt)
+        sol.adapt_dt()
+    torch.cuda.synchronize()
+    _lib.call("sod_profile_enable", 1)
+    _lib.call("sod_profile_reset")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sol.rk_4_main(sol.dt)
+        sol.adapt_dt()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    _lib.call("sod_profile_enable", 0)
+    K = args.steps
+    stages = 4 * K
+    gdofs = 5.0 * m.npoin_w * stages / el / 1e9
+    ek = sol.volAvg_EK()
+    log(f"{K} steps in {el:.3f}s, EK={ek:.6f}, dt={sol.dt:.4e}")
+
+    # dominant kernel and its algorithmic bytes per launch
+    kernels = {}
+    for name in ("full_convec", "full_diffusion", "scalar_convec", "rk4_step"):
+        ms, cnt = kernel_ms(name)
+        if cnt:
+            kernels[name] = (ms, cnt)
+    dom = max((k for k in kernels if k != "rk4_step"), key=lambda k: kernels[k][0])
+    ms, cnt = kernels[dom]
+    Ne, Nn = m.nelem, m.npoin_w
+    # per element: connec 64*4 + He 576*8 + gpvol 64*8; per node: gathered
+    # inputs and scattered outputs of the routine, 8 bytes each
+    per_node = {"full_convec": 9 + 5, "full_diffusion": 6 + 5, "scalar_convec": 7 + 1}[dom]
+    per_elem = 64 * 4 + 576 * 8 + 64 * 8 + (64 * 16 if dom == "full_diffusion" else 0)
+    traffic_alg = Ne * per_elem + Nn * 8 * per_node
+    avg_s = ms / cnt / 1e3
+    achieved = traffic_alg / avg_s / 1e9
+    out = {
+        "metric": "GDOF/s per RK4 stage (FP64, HEX64)",
+        "value": round(gdofs, 4),
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (compressible Taylor-Green vortex initial condition, Re=1600, Mach 0.1)",
+        "config": {"workload": f"tgv_hex64_{n}^3_rk4_cfl0.5", "elements": Ne, "working_nodes": Nn,
+                   "porder": 3, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(ms / cnt, 4), "alg_bytes_per_launch": traffic_alg},
+        "kernels_ms_per_step": {k: round(v[0] / K, 4) for k, v in kernels.items()},
+        "ek_final": ek,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -83,6 +83,14 @@ int sod_update_device(const void *host, size_t bytes);
 int sod_update_host(void *host, size_t bytes);
 void *sod_deviceptr(const void *host);
 
+/* Named kernel timing with HIP events on the library stream; the
+ * counterpart of the reference's NVTX ranges (mod_nvtx.f90:50-101).
+ * sod_profile_get returns the summed duration and count of the ranges
+ * recorded under `name` since the last reset. */
+int sod_profile_enable(int on);
+int sod_profile_reset(void);
+int sod_profile_get(const char *name, double *total_ms, int *count);
+
 /* ------------------------------------------------- init-time tables ---- */
 
 /* GaussLobattoLegendre_hex (quadrature_rules.f90:51-78), hex_highorder /
@@ -95,6 +103,18 @@ void *sod_deviceptr(const void *host);
 int sod_hex64_tables(int *atoIJK, double *xgp, double *wgp, double *Ngp, double *dNgp,
                      double *dlxigp_ip, int *invAtoIJK, int *gmshAtoI, int *gmshAtoJ,
                      int *gmshAtoK);
+
+/* Synthetic input, host-only: the arrays the reference's mesh conversion
+ * tool writes for a structured periodic box of nx*ny*nz HEX64 elements
+ * (reorder_nodes_in_mshRank, mod_meshConversorTool.f90:2864-3076;
+ * create_working_lists_parallel, :3944-4035; GLL node placement, :3921-3942).
+ * elem_order (optional, int64[nelem]) gives the lexicographic element of
+ * each element id.  Outputs: connec_orig(nelem,64), connec(nelem,64)
+ * (= connecParWork), lpoin_w(27*nelem), coord(npoin,3), and optionally
+ * node_lattice[3*npoin] (lattice indices, row-major). */
+int sod_structured_box_mesh(int nx, int ny, int nz, double Lx, double Ly, double Lz,
+                            double curved_amp, const long long *elem_order, int *connec_orig,
+                            int *connec, int *lpoin_w, double *coord, int *node_lattice);
 
 /* elem_jacobian, jacobian_oper.f90:9-138 */
 int sod_elem_jacobian(int nelem, int npoin, const int *connec, const double *coord,
@@ -181,6 +201,26 @@ int sod_volAvg_EK(int nelem, int npoin, const int *connec, const double *gpvol,
  * noBoundaries must be true, isWallModelOn false (or numBoundsWM 0) and
  * flag_les 0; the boundary-only arguments (point2elem, lnbn_nodes, nboun)
  * are accepted and unused, as the reference ignores them in this branch. */
+/* One right-hand-side evaluation at a conserved state (rho, q(npoin,3), E):
+ * the stage EOS, Sutherland viscosity, entropy residual, viscous and
+ * convective residuals that one RK stage of rk_4_main forms
+ * (time_integ.f90:266-373), assembled but not lumped-solved:
+ * Rmass/Rmom/Rener = R_conv + R_diff, Reta = generic_scalar_convec_ijk of
+ * (u eta, eta, u).  No single reference routine computes this sum; it is the
+ * unit the fused stage kernel evaluates, exposed for parity tests and for
+ * the adjoint. */
+int sod_full_rhs(int nelem, int npoin, int npoin_w, const int *connec, const int *lpoin_w,
+                 const double *He, const double *gpvol, const double *Ml, const double *rho,
+                 const double *q, const double *E, const double *mu_factor, const double *mu_e,
+                 const double *mu_sgs, double Rgas, double gamma_gas, double Cp, double Prt,
+                 double *Rmass, double *Rmom, double *Rener, double *Reta);
+
+/* No reference counterpart: selects how sod_rk_4_main evaluates a step.
+ * 0 (default): fused stage kernels (one element kernel + one node kernel
+ * per RK stage); 1: one kernel per reference routine, in the reference's
+ * order.  Both give the reference's results to the stated tolerances. */
+int sod_set_rk4_path(int path);
+
 int sod_rk_4_main(int noBoundaries, int isWallModelOn, int nelem, int nboun, int npoin,
                   int npoin_w, int numBoundsWM, const int *point2elem, const int *lnbn_nodes,
                   const double *dlxigp_ip, const double *xgp, const int *atoIJK,

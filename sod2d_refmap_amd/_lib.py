@@ -47,7 +47,11 @@ _SIGS = {
     "sod_exit_data": [P, c_int],
     "sod_update_device": [P, c_size_t],
     "sod_update_host": [P, c_size_t],
+    "sod_profile_enable": [c_int],
+    "sod_profile_reset": [],
+    "sod_profile_get": [ctypes.c_char_p, ctypes.POINTER(c_double), ctypes.POINTER(c_int)],
     "sod_hex64_tables": [P] * 10,
+    "sod_structured_box_mesh": [c_int] * 3 + [c_double] * 4 + [P] * 6,
     "sod_elem_jacobian": [c_int, c_int] + [P] * 6,
     "sod_lumped_mass_spectral": [c_int, c_int, P, P, P],
     "sod_char_length": [c_int, c_int, P, P, P],
@@ -62,6 +66,8 @@ _SIGS = {
     "sod_smart_visc_spectral": [c_int, c_int, c_int] + [P] * 8 + [c_double] + [P] * 14,
     "sod_adapt_dt_cfl": [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P],
     "sod_volAvg_EK": [c_int, c_int, P, P, P, c_double, P, P, P],
+    "sod_set_rk4_path": [c_int],
+    "sod_full_rhs": [c_int] * 3 + [P] * 11 + [c_double] * 4 + [P] * 4,
     "sod_rk_4_main": ([c_int] * 7 + [P] * 9 + [c_int] + [P] * 8 + [c_double] + [P] * 2 +
                       [c_double] * 4 + [P] * 22),
 }

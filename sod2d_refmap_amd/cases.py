@@ -12,7 +12,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .mesh import Mesh
+from .mesh import Mesh, hex64_tables
 
 
 @dataclass
@@ -66,16 +66,14 @@ def fourier_velocity_perturbation(mesh: Mesh, seed: int, amp: float = 1e-2, kmax
     nx, ny, nz = mesh.n
     Lx, Ly, Lz = mesh.L
     lat = mesh.node_lattice
-    # coordinate of each lattice index along each axis (periodic: x = L maps to 0 phase)
+    r = hex64_tables().gll_roots
+    # unperturbed coordinate of each lattice index along each axis
     axes = []
     for d, (nel, Ld) in enumerate(((nx, Lx), (ny, Ly), (nz, Lz))):
-        M = 3 * nel + 1
-        idx = np.arange(M)
-        xs = np.empty(M)
-        # unperturbed positions from the mesh itself
-        first = np.zeros(M, dtype=np.int64)
-        first[lat[:, d]] = np.arange(lat.shape[0])
-        xs[:] = mesh.x0[first, d]
+        idx = np.arange(3 * nel + 1)
+        h = Ld / nel
+        e = np.minimum(idx // 3, nel - 1)
+        xs = e * h + (1.0 + r[idx - 3 * e]) * (0.5 * h)
         axes.append(np.exp(1j * np.outer(ks, 2 * np.pi * xs / Ld)))   # (2km+1, M)
     out = np.zeros((mesh.npoin, 3))
     for c in range(3):

@@ -475,6 +475,7 @@ void launch_full_convec(bool total_h, int nelem, int npoin, const int* connec, c
                         const double* pr, const double* E, double* Rmass, double* Rmom,
                         double* Rener) {
     if (nelem <= 0) return;
+    SOD_PROF("full_convec");
     if (total_h)
         hipLaunchKernelGGL(k_full_convec<true>, dim3(grid_for(nelem, EPB)), dim3(64 * EPB), 0,
                            rt().stream, nelem, npoin, connec, He, gpvol, u, q, rho, pr, E, Rmass,
@@ -489,6 +490,7 @@ void launch_scalar_convec(int nelem, int npoin, const int* connec, const double*
                           const double* gpvol, const double* q, const double* eta, const double* u,
                           double* R) {
     if (nelem <= 0) return;
+    SOD_PROF("scalar_convec");
     hipLaunchKernelGGL(k_scalar_convec, dim3(grid_for(nelem, EPB)), dim3(64 * EPB), 0, rt().stream,
                        nelem, npoin, connec, He, gpvol, q, eta, u, R);
 }
@@ -499,6 +501,7 @@ void launch_full_diffusion(int nelem, int npoin, const int* connec, const double
                            const double* mu_fluid, const double* mu_e, const double* mu_sgs,
                            double* Rmass, double* Rmom, double* Rener) {
     if (nelem <= 0) return;
+    SOD_PROF("full_diffusion");
     const sod_numerical_params& p = rt().params;
     hipLaunchKernelGGL(k_full_diffusion, dim3(grid_for(nelem, EPB)), dim3(64 * EPB), 0, rt().stream,
                        nelem, npoin, connec, He, gpvol, Cp, Pr, p.c_rho, p.c_ener, rho_n, rho, u,

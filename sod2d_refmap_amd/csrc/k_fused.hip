@@ -1,0 +1,683 @@
+// Fused RK4 stage for rk_4_main (time_integ.f90:234-411): one kernel per
+// stage evaluates, element by element, the stage equation of state
+// (:266-285), Sutherland viscosity (:312-316), the entropy residual
+// (generic_scalar_convec_ijk), the viscous residual (full_diffusion_ijk) and
+// the split-form convective residual (full_convec_ijk) and scatters the six
+// residuals; one node kernel per stage does the lumped solve (:391-393), the
+// accumulation (:399-409) and the next stage state (:239-248) or, after the
+// last stage, the update and final equation of state (:417-471).
+//
+// Data layout (built once per mesh, "prepared mesh"):
+//   conn  [e][64]    int32 node id (0-based) in lattice-lane order
+//   He    [e][9][64] He(i,j) at row i+3j, lane order
+//   gpvol [e][64], mu_e [e][64], mu_sgs [e][64]  lane order
+// Nodal arrays keep the reference's SoA layout.
+//
+// Work decomposition: one wavefront per element, lane = lattice node
+// (i + 4j + 16k).  Nodal values are gathered once into LDS with a swizzled
+// slot (conflict-free 4-point line reads along xi, eta and zeta); each lane
+// then forms its derivatives from the 12 points of the three lines through
+// its node.  Divergences of products a*u are evaluated with the contravariant
+// velocity U^c_b = sum_j He(j,c)(node) u_j(b), and advective terms u.grad(phi)
+// with the node's own contravariant velocity, which is the same discrete
+// operator as the reference's (He applied to the xi-derivative of each product)
+// rearranged by linearity.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "sod2d_internal.hpp"
+
+namespace sod {
+
+namespace {
+
+constexpr int WPB = 4;   // wavefronts (elements) per workgroup
+constexpr int BLK = 256;
+
+__device__ __forceinline__ int swz(int i, int j, int k) {
+    return 16 * k + ((i + k) & 3) + 4 * ((j + k) & 3);
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct StageParams {
+    double gamma, Rgas, Cp, Pr, c_rho, c_ener;
+};
+
+enum { P_RHO, P_U0, P_U1, P_U2, P_P, P_H, P_K, P_ETA, P_T, NPRIM };
+enum { R_MASS, R_MOM0, R_MOM1, R_MOM2, R_ENER, R_ETA, NRES };
+
+// ------------------------------------------------------------ element kernel
+template <bool HAS_SGS>
+__global__ __launch_bounds__(64 * WPB) void k_stage_rhs(
+    int nelem, int npoin, const int* __restrict__ conn, const double* __restrict__ He,
+    const double* __restrict__ gpv, const double* __restrict__ mue, const double* __restrict__ musgs,
+    const double* __restrict__ rho, const double* __restrict__ q0, const double* __restrict__ q1,
+    const double* __restrict__ q2, const double* __restrict__ Ein, const double* __restrict__ mufac,
+    double* __restrict__ R, StageParams P) {
+    __shared__ double sh[WPB][18][64];
+    // The 4x4 derivative matrix is staged in LDS and indexed flat: indexing
+    // the __constant__ 2-D table with a lane-dependent row gave wrong
+    // addresses for this kernel with the current compiler.
+    __shared__ double sD[16];
+    if (threadIdx.x < 16) sD[threadIdx.x] = (&c_tab.D[0][0])[threadIdx.x];
+    __syncthreads();
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int e = blockIdx.x * WPB + w;
+    if (e >= nelem) return;   // uniform per wavefront; only wave-level sync below
+    const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
+    const size_t eb = (size_t)e * 64;
+    const size_t np = (size_t)npoin;
+
+    // ---- gather + equation of state (time_integ.f90:266-285) -----------
+    const int n = conn[eb + l];
+    const double r = rho[n];
+    const double m0 = q0[n], m1 = q1[n], m2 = q2[n], en = Ein[n], mf = mufac[n];
+    const double u0 = m0 / r, u1 = m1 / r, u2 = m2 / r;
+    const double umag = u0 * u0 + u1 * u1 + u2 * u2;
+    const double eint = (en / r) - 0.5 * umag;
+    const double g = P.gamma;
+    const double p = r * (g - 1.0) * eint;
+    const double h = (g / (g - 1.0)) * p / r;
+    const double T = p / (r * P.Rgas);
+    const double eta = (r / (g - 1.0)) * log(p / pow(r, g));
+    const double kk = 0.5 * umag;
+    const double muf = sutherland(mf, T);
+    double hm[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) hm[d][c] = He[eb * 9 + (d + 3 * c) * 64 + l];
+    const double gv = gpv[eb + l];
+    const double mu_e = mue[eb + l];
+    const double mu_s = HAS_SGS ? musgs[eb + l] : 0.0;
+
+    double(*S)[64] = sh[w];
+    const int me = swz(i, j, k);
+    S[P_RHO][me] = r;
+    S[P_U0][me] = u0;
+    S[P_U1][me] = u1;
+    S[P_U2][me] = u2;
+    S[P_P][me] = p;
+    S[P_H][me] = h;
+    S[P_K][me] = kk;
+    S[P_ETA][me] = eta;
+    S[P_T][me] = T;
+    wave_sync();
+
+    // ---- directional passes ---------------------------------------------
+    double divF0 = 0, divF1 = 0, divF2 = 0, divU0 = 0, divU1 = 0, divU2 = 0;
+    double divFe = 0, divFue = 0, divFk = 0, divFuk = 0, divFeta = 0, divQ = 0;
+    double Ah = 0, Arh = 0, Ak = 0, Ark = 0, Aq0 = 0, Aq1 = 0, Aq2 = 0, Aeta = 0;
+    double gP[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, gR[3] = {0, 0, 0};
+    double gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};   // gU[a][b] = du_a/dx_b
+    const int own[3] = {i, j, k};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double h0 = hm[0][c], h1 = hm[1][c], h2 = hm[2][c];
+        const double Ua = h0 * u0 + h1 * u1 + h2 * u2;
+        double dp = 0, du0 = 0, du1 = 0, du2 = 0, dT = 0, dr = 0;
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+            const int sl = c == 0 ? swz(bb, j, k) : (c == 1 ? swz(i, bb, k) : swz(i, j, bb));
+            const double Dc = sD[own[c] * 4 + bb];
+            const double rb = S[P_RHO][sl], u0b = S[P_U0][sl], u1b = S[P_U1][sl], u2b = S[P_U2][sl];
+            const double pb = S[P_P][sl], hb = S[P_H][sl], kb = S[P_K][sl], etab = S[P_ETA][sl],
+                         Tb = S[P_T][sl];
+            dp += Dc * pb;
+            du0 += Dc * u0b;
+            du1 += Dc * u1b;
+            du2 += Dc * u2b;
+            dT += Dc * Tb;
+            dr += Dc * rb;
+            const double DU = Dc * (h0 * u0b + h1 * u1b + h2 * u2b);
+            const double rhb = rb * hb, rkb = rb * kb;
+            const double q0b = rb * u0b, q1b = rb * u1b, q2b = rb * u2b;
+            divF0 += q0b * DU;
+            divF1 += q1b * DU;
+            divF2 += q2b * DU;
+            divU0 += u0b * DU;
+            divU1 += u1b * DU;
+            divU2 += u2b * DU;
+            divFe += rhb * DU;
+            divFue += hb * DU;
+            divFk += rkb * DU;
+            divFuk += kb * DU;
+            divFeta += etab * DU;
+            divQ += rb * DU;
+            const double DA = Dc * Ua;
+            Ah += DA * hb;
+            Arh += DA * rhb;
+            Ak += DA * kb;
+            Ark += DA * rkb;
+            Aq0 += DA * q0b;
+            Aq1 += DA * q1b;
+            Aq2 += DA * q2b;
+            Aeta += DA * etab;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double hd = hm[d][c];
+            gP[d] += hd * dp;
+            gT[d] += hd * dT;
+            gR[d] += hd * dr;
+            gU[0][d] += hd * du0;
+            gU[1][d] += hd * du1;
+            gU[2][d] += hd * du2;
+        }
+    }
+
+    // ---- convective residual (elem_convec.f90:414-434) -------------------
+    const double uu[3] = {u0, u1, u2};
+    const double qq[3] = {r * u0, r * u1, r * u2};
+    const double divU = gU[0][0] + gU[1][1] + gU[2][2];
+    const double Arho = u0 * gR[0] + u1 * gR[1] + u2 * gR[2];
+    const double divF[3] = {divF0, divF1, divF2}, divFuu[3] = {divU0, divU1, divU2};
+    const double Aq[3] = {Aq0, Aq1, Aq2};
+    double Rm = 0.5 * (divQ + r * divU) + 0.5 * Arho;
+    double Re = 0.25 * (divFe + h * divQ + (r * h) * divU + r * divFue) +
+                0.25 * (divFk + kk * divQ + (r * kk) * divU + r * divFuk) +
+                0.25 * (r * Ah + Arh + h * Arho) + 0.25 * (r * Ak + Ark + kk * Arho);
+    double Rq[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double Au = u0 * gU[a][0] + u1 * gU[a][1] + u2 * gU[a][2];
+        Rq[a] = 0.25 * (divF[a] + uu[a] * divQ + qq[a] * divU + r * divFuu[a]) +
+                0.25 * (uu[a] * Arho + r * Au + Aq[a]) + gP[a];
+    }
+    const double Reta = 0.5 * (divFeta + eta * divU) + 0.5 * Aeta;
+
+    // ---- viscous fluxes at this node (elem_diffu.f90:70-140) -------------
+    const double mu_fgp = muf + r * mu_s;
+    const double mu_t = mu_fgp + mu_e;
+    const double twoThirds = 2.0 / 3.0;
+    double tau[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) tau[a][b] = mu_t * (gU[a][b] + gU[b][a]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) tau[a][a] = tau[a][a] - mu_fgp * twoThirds * divU;
+    double tauU[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) tauU[a] = tau[a][0] * u0 + tau[a][1] * u1 + tau[a][2] * u2;
+    // contravariant fluxes G_c = gpvol * sum_d He(d,c) F_d for the six vector
+    // fluxes grad(rho), tau rows 0..2, tau.u, grad(T)
+    wave_sync();   // all lanes are done reading the nodal values
+    {
+        const double* F[6] = {gR, tau[0], tau[1], tau[2], tauU, gT};
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                S[3 * f + c][me] = gv * (hm[0][c] * F[f][0] + hm[1][c] * F[f][1] + hm[2][c] * F[f][2]);
+    }
+    wave_sync();
+    // ---- weak divergence at this test node (elem_diffu.f90:142-194) ------
+    double dDr = 0, dDm0 = 0, dDm1 = 0, dDm2 = 0, dDtu = 0, dDT = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+            const int sl = c == 0 ? swz(bb, j, k) : (c == 1 ? swz(i, bb, k) : swz(i, j, bb));
+            const double Dt = sD[bb * 4 + own[c]];   // l'_own(xi_q)
+            dDr += Dt * S[0 + c][sl];
+            dDm0 += Dt * S[3 + c][sl];
+            dDm1 += Dt * S[6 + c][sl];
+            dDm2 += Dt * S[9 + c][sl];
+            dDtu += Dt * S[12 + c][sl];
+            dDT += Dt * S[15 + c][sl];
+        }
+    }
+    const double nu_e = P.c_rho * mu_e / r;
+    const double kappa = muf * P.Cp / P.Pr + P.c_ener * mu_e * P.Cp / P.Pr + P.Cp * r * mu_s / 0.9;
+
+    double* Rn = R + n;
+    unsafeAtomicAdd(Rn + R_MASS * np, gv * Rm + nu_e * dDr);
+    unsafeAtomicAdd(Rn + R_MOM0 * np, gv * Rq[0] + dDm0);
+    unsafeAtomicAdd(Rn + R_MOM1 * np, gv * Rq[1] + dDm1);
+    unsafeAtomicAdd(Rn + R_MOM2 * np, gv * Rq[2] + dDm2);
+    unsafeAtomicAdd(Rn + R_ENER * np, gv * Re + dDtu + kappa * dDT);
+    unsafeAtomicAdd(Rn + R_ETA * np, gv * Reta);
+}
+
+// ------------------------------------------------------------ node kernels
+__constant__ double k_a[4] = {0.0, 0.5, 0.5, 1.0};
+__constant__ double k_b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+
+struct StepState {
+    // caller level-2 arrays
+    double *rho, *q0, *q1, *q2, *E;
+    double *u, *pr, *Tem, *e_int, *eta, *csound, *machno, *mu_fluid;
+    const double *Ml, *mu_factor;
+    // library arrays
+    double *aux;    // 5 x npoin: rho, q0, q1, q2, E of the next stage
+    double *R;      // 6 x npoin
+    double *Rsum;   // 6 x npoin
+    double *partial;   // reduction partials (RED_BLOCKS)
+};
+
+constexpr int RED_BLOCKS = 1024;
+
+// Stage s in 0..3 after its element kernel: lumped solve, accumulation,
+// next stage state (s < 3) or final update + EOS (s == 3).
+template <bool FINAL>
+__global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, const int* __restrict__ lpoin_w,
+                                                      int s, double dt, double gamma, double Rgas,
+                                                      StepState st) {
+    __shared__ double s_red[BLK / 64];
+    const size_t np = (size_t)npoin;
+    double eta_acc = 0.0;
+    for (int it = blockIdx.x * BLK + threadIdx.x; it < npoin_w; it += gridDim.x * BLK) {
+        const int n = lpoin_w[it] - 1;
+        const double ml = st.Ml[n];
+        double Rf[NRES];
+#pragma unroll
+        for (int f = 0; f < NRES; ++f) {
+            Rf[f] = st.R[f * np + n] / ml;   // mod_solver.f90:32-36
+            st.R[f * np + n] = 0.0;          // ready for the next stage
+        }
+        const double b = k_b[s];
+        double Rs[NRES];
+#pragma unroll
+        for (int f = 0; f < NRES; ++f) {
+            const double prev = s == 0 ? 0.0 : st.Rsum[f * np + n];
+            Rs[f] = prev + b * Rf[f];
+        }
+        const double y[5] = {st.rho[n], st.q0[n], st.q1[n], st.q2[n], st.E[n]};
+        // residual order: mass, mom0..2, ener -> state order rho, q0..2, E
+        if (!FINAL) {
+#pragma unroll
+            for (int f = 0; f < NRES; ++f) st.Rsum[f * np + n] = Rs[f];
+            const double fa = dt * k_a[s + 1];
+#pragma unroll
+            for (int f = 0; f < 5; ++f) st.aux[f * np + n] = y[f] - fa * Rf[f];
+        } else {
+            double yn[5];
+#pragma unroll
+            for (int f = 0; f < 5; ++f) yn[f] = y[f] - dt * Rs[f];
+            st.rho[n] = yn[0];
+            st.q0[n] = yn[1];
+            st.q1[n] = yn[2];
+            st.q2[n] = yn[3];
+            st.E[n] = yn[4];
+            // final EOS (time_integ.f90:445-463)
+            const double r = yn[0];
+            const double uv[3] = {yn[1] / r, yn[2] / r, yn[3] / r};
+            double umag = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                st.u[n + np * d] = uv[d];
+                umag = umag + uv[d] * uv[d];
+            }
+            const double ei = (yn[4] / r) - 0.5 * umag;
+            const double p = r * (gamma - 1.0) * ei;
+            st.e_int[n] = ei;
+            st.pr[n] = p;
+            const double c = sqrt(gamma * p / r);
+            st.csound[n] = c;
+            st.machno[n] = sqrt(umag) / c;
+            const double T = p / (r * Rgas);
+            st.Tem[n] = T;
+            const double et = (r / (gamma - 1.0)) * log(p / pow(r, gamma));
+            st.eta[n] = et;
+            eta_acc += et;
+            // entropy residual Reta = -Reta_sum (time_integ.f90:467-471),
+            // kept in the (now free) first aux array for the entropy viscosity
+            st.aux[n] = -Rs[R_ETA];
+            st.mu_fluid[n] = sutherland(st.mu_factor[n], T);
+        }
+    }
+    if (FINAL) {
+        // partial sums of eta over working nodes for the normalisation
+        double v = eta_acc;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int q = 0; q < BLK / 64; ++q) t += s_red[q];
+            st.partial[blockIdx.x] = t;
+        }
+    }
+}
+
+// mu_fluid at periodic slave nodes: Sutherland of their (unchanged) Tem,
+// as the end-of-step call over all npoin does (time_integ.f90:482-486).
+__global__ __launch_bounds__(BLK) void k_slave_sutherland(int nslave, const int* __restrict__ slaves,
+                                                          const double* __restrict__ Tem,
+                                                          const double* __restrict__ mu_factor,
+                                                          double* __restrict__ mu_fluid) {
+    const int t = blockIdx.x * BLK + threadIdx.x;
+    if (t >= nslave) return;
+    const int n = slaves[t];
+    mu_fluid[n] = sutherland(mu_factor[n], Tem[n]);
+}
+
+// reference-layout element-node array -> lane layout; flags nonzero values
+__global__ __launch_bounds__(BLK) void k_elemnode_to_lane(int nelem, const double* __restrict__ src,
+                                                          double* __restrict__ dst,
+                                                          int* __restrict__ nonzero) {
+    const long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)nelem * 64) return;
+    const int e = (int)(t >> 6), l = (int)(t & 63);
+    const int a = c_tab.lane2node[l];
+    const double v = src[(size_t)e + (size_t)nelem * a];
+    dst[t] = v;
+    if (nonzero && v != 0.0) *nonzero = 1;
+}
+
+// mesh preparation: connec (ref layout, 1-based) -> lane layout (0-based),
+// He and gpvol -> lane layout
+__global__ __launch_bounds__(BLK) void k_prepare_mesh(int nelem, const int* __restrict__ connec,
+                                                      const double* __restrict__ He,
+                                                      const double* __restrict__ gpvol,
+                                                      int* __restrict__ conn_l, double* __restrict__ He_l,
+                                                      double* __restrict__ gpv_l) {
+    const long t = (long)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (long)nelem * 64) return;
+    const int e = (int)(t >> 6), l = (int)(t & 63);
+    const int a = c_tab.lane2node[l];
+    conn_l[t] = connec[(size_t)e + (size_t)nelem * a] - 1;
+    for (int c = 0; c < 9; ++c) He_l[(size_t)e * 576 + c * 64 + l] = He[9 * (size_t)a + 576 * (size_t)e + c];
+    gpv_l[t] = gpvol[a + 64 * (size_t)e];
+}
+
+__global__ void k_zero_int(int* p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+// Position-weighted fingerprint of sampled mesh data (connec, He, gpvol,
+// lpoin_w, Ml) so that a prepared mesh is rebuilt when a caller reuses the
+// same addresses for different contents.
+__global__ __launch_bounds__(256) void k_fingerprint(int nelem, int npoin_w, const int* __restrict__ connec,
+                                                     const double* __restrict__ He,
+                                                     const double* __restrict__ gpvol,
+                                                     const int* __restrict__ lpoin_w,
+                                                     const double* __restrict__ Ml, double* __restrict__ out) {
+    __shared__ double s[4][256];
+    const int t = threadIdx.x;
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int smp = 0; smp < 16; ++smp) {
+        const int e = (int)(((long)nelem * smp + (t >> 4)) / 16) % nelem;
+        const int nd = t & 63;
+        const double wgt = 1.0 + 0.001 * (t + 256 * smp);
+        a += wgt * connec[(size_t)e + (size_t)nelem * nd];
+        b += wgt * He[576 * (size_t)e + (t % 576)];
+        c += wgt * gpvol[64 * (size_t)e + nd];
+        const int w = (int)(((long)npoin_w * (t + 256 * smp)) / 4096) % npoin_w;
+        d += wgt * lpoin_w[w] + wgt * Ml[lpoin_w[w] - 1];
+    }
+    s[0][t] = a;
+    s[1][t] = b;
+    s[2][t] = c;
+    s[3][t] = d;
+    __syncthreads();
+    if (t < 4) {
+        double v = 0;
+        for (int q = 0; q < 256; ++q) v += s[t][q];
+        out[t] = v;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ prepared mesh
+struct Prepared {
+    const void* key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int nelem = -1, npoin = -1, npoin_w = -1;
+    int* conn = nullptr;
+    double* He = nullptr;
+    double* gpv = nullptr;
+    double* mue = nullptr;
+    double* musgs = nullptr;
+    int* slaves = nullptr;
+    int nslave = 0;
+    double* work = nullptr;   // aux(5) + R(6) + Rsum(6) per npoin, + partials
+    int* flag = nullptr;
+    size_t work_doubles = 0;
+    void release() {
+        hipFree(conn);
+        hipFree(He);
+        hipFree(gpv);
+        hipFree(mue);
+        hipFree(musgs);
+        hipFree(slaves);
+        hipFree(work);
+        hipFree(flag);
+        *this = Prepared();
+    }
+};
+
+static Prepared g_prep;
+
+void fused_release() { g_prep.release(); }
+
+static double* g_fpbuf = nullptr;
+static double g_fp[4] = {0, 0, 0, 0};
+
+static int prepare(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
+                   const double* He, const double* gpvol, const double* Ml) {
+    Prepared& P = g_prep;
+    const void* key[6] = {connec, lpoin_w, He, gpvol, Ml, nullptr};
+    bool same = P.nelem == nelem && P.npoin == npoin && P.npoin_w == npoin_w;
+    for (int t = 0; t < 5 && same; ++t) same = P.key[t] == key[t];
+    double fp[4] = {0, 0, 0, 0};
+    if (nelem > 0 && npoin_w > 0) {
+        if (!g_fpbuf) SOD_HIP(hipMalloc(&g_fpbuf, 4 * sizeof(double)));
+        hipLaunchKernelGGL(k_fingerprint, dim3(1), dim3(256), 0, rt().stream, nelem, npoin_w, connec, He, gpvol,
+                           lpoin_w, Ml, g_fpbuf);
+        SOD_HIP(hipMemcpyAsync(fp, g_fpbuf, sizeof(fp), hipMemcpyDeviceToHost, rt().stream));
+        SOD_HIP(hipStreamSynchronize(rt().stream));
+    }
+    if (same && std::memcmp(fp, g_fp, sizeof(fp)) == 0) return SOD_OK;
+    std::memcpy(g_fp, fp, sizeof(fp));
+    SOD_HIP(hipStreamSynchronize(rt().stream));
+    P.release();
+    const size_t ne = (size_t)nelem, np = (size_t)npoin;
+    SOD_HIP(hipMalloc(&P.conn, sizeof(int) * ne * 64 + 4));
+    SOD_HIP(hipMalloc(&P.He, sizeof(double) * ne * 576 + 8));
+    SOD_HIP(hipMalloc(&P.gpv, sizeof(double) * ne * 64 + 8));
+    SOD_HIP(hipMalloc(&P.mue, sizeof(double) * ne * 64 + 8));
+    SOD_HIP(hipMalloc(&P.musgs, sizeof(double) * ne * 64 + 8));
+    SOD_HIP(hipMalloc(&P.flag, sizeof(int) * 4));
+    P.work_doubles = np * 17 + 2 * RED_BLOCKS + 64;
+    SOD_HIP(hipMalloc(&P.work, sizeof(double) * P.work_doubles));
+    SOD_HIP(hipMemsetAsync(P.work, 0, sizeof(double) * P.work_doubles, rt().stream));
+    if (nelem > 0)
+        hipLaunchKernelGGL(k_prepare_mesh, dim3(grid_for((long)nelem * 64, BLK)), dim3(BLK), 0, rt().stream,
+                           nelem, connec, He, gpvol, P.conn, P.He, P.gpv);
+    // slave list: nodes that are not in lpoin_w
+    std::vector<int> hw(npoin_w);
+    SOD_HIP(hipStreamSynchronize(rt().stream));
+    SOD_HIP(hipMemcpy(hw.data(), lpoin_w, sizeof(int) * npoin_w, hipMemcpyDeviceToHost));
+    std::vector<char> isw(npoin, 0);
+    for (int t = 0; t < npoin_w; ++t) {
+        if (hw[t] < 1 || hw[t] > npoin) return fail(SOD_ERR_ARG, "lpoin_w holds an id outside 1..npoin");
+        isw[hw[t] - 1] = 1;
+    }
+    std::vector<int> sl;
+    for (int t = 0; t < npoin; ++t)
+        if (!isw[t]) sl.push_back(t);
+    P.nslave = (int)sl.size();
+    SOD_HIP(hipMalloc(&P.slaves, sizeof(int) * (sl.size() + 1)));
+    if (!sl.empty()) SOD_HIP(hipMemcpy(P.slaves, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice));
+    SOD_HIP(hipGetLastError());
+    for (int t = 0; t < 5; ++t) P.key[t] = key[t];
+    P.nelem = nelem;
+    P.npoin = npoin;
+    P.npoin_w = npoin_w;
+    return SOD_OK;
+}
+
+void launch_smart_visc_fused(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
+                             const double* Reta, const double* gpvol, double gamma_gas, const double* rho,
+                             const double* u, const double* csound, const double* Tem,
+                             const double* helem_l, const double* helem, double* mu_e, double* mue_l,
+                             const double* eta, const double* partial, int npartial, double* work);
+
+int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
+                   const double* He, const double* gpvol, const double* Ml, const double* helem,
+                   const double* helem_l, double dt, double Rgas, double gamma_gas, double Cp, double Prt,
+                   const double* mu_factor, double* rho, double* u, double* q, double* pr, double* E,
+                   double* Tem, double* e_int, double* eta, double* csound, double* machno,
+                   double* mu_fluid, double* mu_e, const double* mu_sgs, double* mue_l) {
+    SOD_TRY(prepare(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
+    Prepared& Pm = g_prep;
+    hipStream_t s = rt().stream;
+    const sod_numerical_params& prm = rt().params;
+    const size_t np = (size_t)npoin;
+    // element-node viscosities in lane layout; mu_sgs enters only if non-zero
+    int has_sgs = 0;
+    if (nelem > 0) {
+        const dim3 g(grid_for((long)nelem * 64, BLK));
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, Pm.flag);
+        hipLaunchKernelGGL(k_elemnode_to_lane, g, dim3(BLK), 0, s, nelem, mu_e, Pm.mue, (int*)nullptr);
+        hipLaunchKernelGGL(k_elemnode_to_lane, g, dim3(BLK), 0, s, nelem, mu_sgs, Pm.musgs, Pm.flag);
+        SOD_HIP(hipMemcpyAsync(&has_sgs, Pm.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        SOD_HIP(hipStreamSynchronize(s));
+    }
+    StepState st;
+    st.rho = rho;
+    st.q0 = q;
+    st.q1 = q + np;
+    st.q2 = q + 2 * np;
+    st.E = E;
+    st.u = u;
+    st.pr = pr;
+    st.Tem = Tem;
+    st.e_int = e_int;
+    st.eta = eta;
+    st.csound = csound;
+    st.machno = machno;
+    st.mu_fluid = mu_fluid;
+    st.Ml = Ml;
+    st.mu_factor = mu_factor;
+    st.aux = Pm.work;
+    st.R = Pm.work + 5 * np;
+    st.Rsum = Pm.work + 11 * np;
+    st.partial = Pm.work + 17 * np;
+    StageParams sp{gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener};
+    const int ugrid = std::min(RED_BLOCKS, std::max(1, grid_for(npoin_w, BLK)));
+    for (int stage = 0; stage < 4; ++stage) {
+        const double *ir, *i0, *i1, *i2, *iE;
+        if (stage == 0) {
+            ir = st.rho; i0 = st.q0; i1 = st.q1; i2 = st.q2; iE = st.E;
+        } else {
+            ir = st.aux; i0 = st.aux + np; i1 = st.aux + 2 * np; i2 = st.aux + 3 * np; iE = st.aux + 4 * np;
+        }
+        if (nelem > 0) {
+            SOD_PROF("stage_rhs");
+            if (has_sgs)
+                hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem,
+                                   npoin, Pm.conn, Pm.He, Pm.gpv, Pm.mue, Pm.musgs, ir, i0, i1, i2, iE,
+                                   mu_factor, st.R, sp);
+            else
+                hipLaunchKernelGGL(k_stage_rhs<false>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem,
+                                   npoin, Pm.conn, Pm.He, Pm.gpv, Pm.mue, Pm.musgs, ir, i0, i1, i2, iE,
+                                   mu_factor, st.R, sp);
+        }
+        SOD_PROF("stage_update");
+        if (stage < 3)
+            hipLaunchKernelGGL(k_stage_update<false>, dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w, lpoin_w, stage,
+                               dt, gamma_gas, Rgas, st);
+        else
+            hipLaunchKernelGGL(k_stage_update<true>, dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w, lpoin_w, stage,
+                               dt, gamma_gas, Rgas, st);
+    }
+    if (Pm.nslave > 0)
+        hipLaunchKernelGGL(k_slave_sutherland, dim3(grid_for(Pm.nslave, BLK)), dim3(BLK), 0, s, Pm.nslave,
+                           Pm.slaves, Tem, mu_factor, mu_fluid);
+    // entropy viscosity for the next step (time_integ.f90:492-493)
+    {
+        SOD_PROF("smart_visc_fused");
+        launch_smart_visc_fused(nelem, npoin, npoin_w, connec, lpoin_w, st.aux, gpvol, gamma_gas, rho,
+                                u, csound, Tem, helem_l, helem, mu_e, mue_l, eta, st.partial, ugrid,
+                                st.partial + RED_BLOCKS);
+    }
+    SOD_HIP(hipGetLastError());
+    return SOD_OK;
+}
+
+// One evaluation of the full right-hand side at a conserved state: the
+// assembled (not yet lumped-solved) residuals R_conv + R_diff of mass,
+// momentum and energy and the entropy residual, as rk_4_main forms them in
+// one stage (time_integ.f90:266-373).  Out arrays are overwritten.
+int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w, const double* He,
+             const double* gpvol, const double* Ml, const double* rho, const double* q, const double* E,
+             const double* mu_factor, const double* mu_e, const double* mu_sgs, double Rgas, double gamma_gas,
+             double Cp, double Prt, double* Rmass, double* Rmom, double* Rener, double* Reta) {
+    SOD_TRY(prepare(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
+    Prepared& Pm = g_prep;
+    hipStream_t s = rt().stream;
+    const sod_numerical_params& prm = rt().params;
+    const size_t np = (size_t)npoin;
+    void* buf = nullptr;
+    SOD_TRY(scratch(sizeof(double) * np * NRES, &buf, 5));
+    double* R = static_cast<double*>(buf);
+    SOD_HIP(hipMemsetAsync(R, 0, sizeof(double) * np * NRES, s));
+    int has_sgs = 0;
+    if (nelem > 0) {
+        const dim3 g(grid_for((long)nelem * 64, BLK));
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, Pm.flag);
+        hipLaunchKernelGGL(k_elemnode_to_lane, g, dim3(BLK), 0, s, nelem, mu_e, Pm.mue, (int*)nullptr);
+        hipLaunchKernelGGL(k_elemnode_to_lane, g, dim3(BLK), 0, s, nelem, mu_sgs, Pm.musgs, Pm.flag);
+        SOD_HIP(hipMemcpyAsync(&has_sgs, Pm.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        SOD_HIP(hipStreamSynchronize(s));
+        StageParams sp{gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener};
+        if (has_sgs)
+            hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, npoin,
+                               Pm.conn, Pm.He, Pm.gpv, Pm.mue, Pm.musgs, rho, q, q + np, q + 2 * np, E, mu_factor,
+                               R, sp);
+        else
+            hipLaunchKernelGGL(k_stage_rhs<false>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, npoin,
+                               Pm.conn, Pm.He, Pm.gpv, Pm.mue, Pm.musgs, rho, q, q + np, q + 2 * np, E, mu_factor,
+                               R, sp);
+    }
+    SOD_HIP(hipMemcpyAsync(Rmass, R + R_MASS * np, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+    SOD_HIP(hipMemcpyAsync(Rmom, R + R_MOM0 * np, sizeof(double) * np * 3, hipMemcpyDeviceToDevice, s));
+    SOD_HIP(hipMemcpyAsync(Rener, R + R_ENER * np, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+    SOD_HIP(hipMemcpyAsync(Reta, R + R_ETA * np, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+    SOD_HIP(hipGetLastError());
+    return SOD_OK;
+}
+
+}  // namespace sod
+
+using namespace sod;
+
+extern "C" int sod_full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
+                            const double* He, const double* gpvol, const double* Ml, const double* rho,
+                            const double* q, const double* E, const double* mu_factor, const double* mu_e,
+                            const double* mu_sgs, double Rgas, double gamma_gas, double Cp, double Prt,
+                            double* Rmass, double* Rmom, double* Rener, double* Reta) {
+    SOD_TRY(ensure_init());
+    if (nelem < 0 || npoin < 0 || npoin_w < 0 || npoin_w > npoin) return fail(SOD_ERR_ARG, "full_rhs: sizes");
+    const int *dc, *dl;
+    const double *dHe, *dgv, *dMl, *dr, *dq, *dE, *dmf, *dmue, *dsgs;
+    double *dRm, *dRq, *dRe, *dRt;
+    SOD_TRY(resolve_t(connec, "connec", &dc));
+    SOD_TRY(resolve_t(lpoin_w, "lpoin_w", &dl));
+    SOD_TRY(resolve_t(He, "He", &dHe));
+    SOD_TRY(resolve_t(gpvol, "gpvol", &dgv));
+    SOD_TRY(resolve_t(Ml, "Ml", &dMl));
+    SOD_TRY(resolve_t(rho, "rho", &dr));
+    SOD_TRY(resolve_t(q, "q", &dq));
+    SOD_TRY(resolve_t(E, "E", &dE));
+    SOD_TRY(resolve_t(mu_factor, "mu_factor", &dmf));
+    SOD_TRY(resolve_t(mu_e, "mu_e", &dmue));
+    SOD_TRY(resolve_t(mu_sgs, "mu_sgs", &dsgs));
+    SOD_TRY(resolve_w(Rmass, "Rmass", &dRm));
+    SOD_TRY(resolve_w(Rmom, "Rmom", &dRq));
+    SOD_TRY(resolve_w(Rener, "Rener", &dRe));
+    SOD_TRY(resolve_w(Reta, "Reta", &dRt));
+    SOD_TRY(full_rhs(nelem, npoin, npoin_w, dc, dl, dHe, dgv, dMl, dr, dq, dE, dmf, dmue, dsgs, Rgas, gamma_gas, Cp,
+                     Prt, dRm, dRq, dRe, dRt));
+    return finish_call();
+}

@@ -376,6 +376,30 @@ void launch_smart_visc(int nelem, int npoin, int npoin_w, const int* connec, con
                            mue_l);
 }
 
+// Entropy viscosity after a fused step: the sum of eta over working nodes
+// arrives as `npartial` block partials; `work` holds >= RED_BLOCKS + 2 doubles.
+void launch_smart_visc_fused(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
+                             const double* Reta, const double* gpvol, double gamma_gas, const double* rho,
+                             const double* u, const double* csound, const double* Tem,
+                             const double* helem_l, const double* helem, double* mu_e, double* mue_l,
+                             const double* eta, const double* partial, int npartial, double* work) {
+    const sod_numerical_params& p = rt().params;
+    hipStream_t s = rt().stream;
+    double* sum = work;
+    double* norm = work + 1;
+    double* dev_partial = work + 2;
+    if (p.flag_normalise_entropy == 1) {
+        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLK), 0, s, npartial, partial, sum);
+        hipLaunchKernelGGL(k_eta_partial_dev, dim3(RED_BLOCKS), dim3(BLK), 0, s, npoin_w, lpoin_w, eta, sum,
+                           dev_partial);
+        hipLaunchKernelGGL(k_final_max, dim3(1), dim3(BLK), 0, s, RED_BLOCKS, dev_partial, norm);
+    }
+    if (nelem > 0)
+        hipLaunchKernelGGL(k_smart_visc, dim3(grid_for(nelem, 4)), dim3(256), 0, s, nelem, npoin, connec, Reta,
+                           gpvol, gamma_gas, rho, u, csound, Tem, helem_l, helem, norm,
+                           p.flag_normalise_entropy == 1, p.Rgas_inf, p.ce, p.cmax, p.cglob, mu_e, mue_l);
+}
+
 void launch_adapt_dt(int nelem, int npoin, const int* connec, const double* helem, const double* u,
                      const double* csound, double cfl_conv, int has_diff, double cfl_diff,
                      const double* mu_fluid, const double* mu_sgs, const double* rho,

@@ -177,6 +177,7 @@ int rk4_step_reference_sequence(int nelem, int npoin, int npoin_w, const int* co
                                 double* pr, double* E, double* Tem, double* e_int, double* eta,
                                 double* csound, double* machno, double* mu_fluid, double* mu_e,
                                 const double* mu_sgs, double* mue_l, double* W, double* vwork) {
+    SOD_PROF("rk4_step");
     hipStream_t s = rt().stream;
     const sod_numerical_params& p = rt().params;
     const size_t np = (size_t)npoin;
@@ -231,9 +232,25 @@ int rk4_step_reference_sequence(int nelem, int npoin, int npoin_w, const int* co
 
 int rk4_work_fields() { return F_COUNT; }
 
+int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
+                   const double* He, const double* gpvol, const double* Ml, const double* helem,
+                   const double* helem_l, double dt, double Rgas, double gamma_gas, double Cp, double Prt,
+                   const double* mu_factor, double* rho, double* u, double* q, double* pr, double* E,
+                   double* Tem, double* e_int, double* eta, double* csound, double* machno,
+                   double* mu_fluid, double* mu_e, const double* mu_sgs, double* mue_l);
+
+// 0: fused stage kernels (default); 1: the reference's kernel sequence
+static int g_rk4_path = 0;
+
 }  // namespace sod
 
 using namespace sod;
+
+extern "C" int sod_set_rk4_path(int path) {
+    if (path != 0 && path != 1) return fail(SOD_ERR_ARG, "sod_set_rk4_path: 0 (fused) or 1 (reference sequence)");
+    g_rk4_path = path;
+    return SOD_OK;
+}
 
 extern "C" int sod_rk_4_main(
     int noBoundaries, int isWallModelOn, int nelem, int nboun, int npoin, int npoin_w,
@@ -292,6 +309,11 @@ extern "C" int sod_rk_4_main(
     double* T2 = dT + np;
     double* ei2 = dei + np;
     double* eta2 = deta + np;
+    if (g_rk4_path == 0) {
+        SOD_TRY(rk4_step_fused(nelem, npoin, npoin_w, dc, dl, dHe, dgv, dMl, dh, dhl, dt, Rgas, gamma_gas, Cp, Prt,
+                               dmf, rho2, u2, q2, pr2, E2, T2, ei2, eta2, dcs, dma, dmuf, dmue, dsgs, dmuel));
+        return finish_call();
+    }
     void *W = nullptr, *vw = nullptr;
     SOD_TRY(scratch(sizeof(double) * np * rk4_work_fields(), &W, 3));
     SOD_TRY(scratch(sizeof(double) * (red_blocks() + 8), &vw, 4));

@@ -14,6 +14,7 @@
 namespace sod {
 
 int upload_tables();   // tables.hip
+void fused_release();  // k_fused.hip
 
 namespace {
 thread_local char g_err[2048] = "";
@@ -208,6 +209,7 @@ int sod_finalize(void) {
         if (s.p) SOD_HIP(hipFree(s.p));
     g_scratch.clear();
     g_tables_ok.clear();
+    fused_release();
     rt().initialised = false;
     return SOD_OK;
 }

@@ -73,12 +73,23 @@ struct DevTables {
 // runs once per device.
 static __constant__ DevTables c_tab;
 using TableUploader = int (*)(const DevTables&);
+using TableDumper = int (*)(DevTables* dev_out, hipStream_t s);
 int register_table_uploader(TableUploader fn);
+int register_table_dumper(TableDumper fn);
 namespace {
 inline int upload_tu_tables(const DevTables& d) {
     return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &d, sizeof(DevTables)) == hipSuccess ? 0 : -2;
 }
+// copies this translation unit's device copy back out (init-time self-check)
+__global__ void k_dump_tu_tables(DevTables* out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = c_tab;
+}
+inline int dump_tu_tables(DevTables* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_dump_tu_tables, dim3(1), dim3(64), 0, s, out);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 [[maybe_unused]] const int tu_tables_registered = register_table_uploader(upload_tu_tables);
+[[maybe_unused]] const int tu_tables_dumper = register_table_dumper(dump_tu_tables);
 }  // namespace
 
 // ------------------------------------------------------------- runtime
@@ -115,6 +126,18 @@ int check_tables(const double* dlxigp_ip, const int* invAtoIJK, const int* gmshA
 
 // Device scratch owned by the runtime, grown on demand.
 int scratch(size_t bytes, void** out, int slot);
+
+// Named timing ranges (HIP events on the library stream), enabled by
+// sod_profile_enable(1).
+int prof_id(const char* name);
+struct ProfScope {
+    int id, rec;
+    explicit ProfScope(int id);
+    ~ProfScope();
+};
+#define SOD_PROF(name)                      \
+    static const int _prof_id = prof_id(name); \
+    ProfScope _prof_scope(_prof_id)
 
 // ----------------------------------------------------------- device math
 __device__ __forceinline__ double sutherland(double mu_factor, double T) {

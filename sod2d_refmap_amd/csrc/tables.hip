@@ -170,6 +170,7 @@ void build_tables() {
 }  // namespace
 
 std::vector<TableUploader>& uploaders();
+std::vector<TableDumper>& dumpers();
 
 const HostTables& host_tables() {
     std::call_once(g_tab_once, build_tables);
@@ -187,6 +188,22 @@ int upload_tables() {
     std::memcpy(d.node2lane, t.node2lane, sizeof(d.node2lane));
     for (TableUploader fn : uploaders())
         if (fn(d) != 0) return fail(SOD_ERR_HIP, "uploading the p=3 tables to the device failed");
+    // self-check: every translation unit reads back exactly what was sent
+    DevTables* dev = nullptr;
+    SOD_HIP(hipMalloc(&dev, sizeof(DevTables)));
+    int bad = -1;
+    for (size_t u = 0; u < dumpers().size() && bad < 0; ++u) {
+        DevTables back;
+        SOD_HIP(hipMemset(dev, 0xff, sizeof(DevTables)));
+        if (dumpers()[u](dev, nullptr) != 0) bad = (int)u;
+        SOD_HIP(hipDeviceSynchronize());
+        SOD_HIP(hipMemcpy(&back, dev, sizeof(DevTables), hipMemcpyDeviceToHost));
+        if (std::memcmp(&back, &d, sizeof(DevTables)) != 0) bad = (int)u;
+    }
+    SOD_HIP(hipFree(dev));
+    if (bad >= 0)
+        return fail(SOD_ERR_HIP, "constant tables of translation unit %d (of %zu) did not reach the device",
+                    bad, dumpers().size());
     return SOD_OK;
 }
 
@@ -195,9 +212,19 @@ std::vector<TableUploader>& uploaders() {
     return v;
 }
 
+std::vector<TableDumper>& dumpers() {
+    static std::vector<TableDumper> v;
+    return v;
+}
+
 int register_table_uploader(TableUploader fn) {
     uploaders().push_back(fn);
     return (int)uploaders().size();
+}
+
+int register_table_dumper(TableDumper fn) {
+    dumpers().push_back(fn);
+    return (int)dumpers().size();
 }
 
 }  // namespace sod

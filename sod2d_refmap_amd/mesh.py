@@ -85,7 +85,6 @@ class Mesh:
     coord: np.ndarray        # (npoin,3) float64 F
     elem_lex: np.ndarray     # (nelem,) lexicographic element of each element id
     node_lattice: np.ndarray = field(repr=False)  # (npoin,3) int lattice index of each node
-    x0: np.ndarray = field(repr=False, default=None)  # (npoin,3) unperturbed coordinates
 
     @property
     def is_slave(self) -> np.ndarray:
@@ -103,63 +102,32 @@ def periodic_box(n, L=2.0 * np.pi, curved_amp: float = 0.0, perm_seed: int | Non
     """
     nx, ny, nz = (n, n, n) if np.isscalar(n) else tuple(n)
     Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else tuple(L)
-    t = hex64_tables()
-    # lattice offsets (0..3) of each gmsh local node
-    li = _S2LAT[t.gmshAtoI - 1]
-    lj = _S2LAT[t.gmshAtoJ - 1]
-    lk = _S2LAT[t.gmshAtoK - 1]
     nelem = nx * ny * nz
     elem_lex = np.arange(nelem, dtype=np.int64)
     if perm_seed is not None:
         elem_lex = np.random.default_rng(perm_seed).permutation(nelem).astype(np.int64)
-    ex = elem_lex % nx
-    ey = (elem_lex // nx) % ny
-    ez = elem_lex // (nx * ny)
-    Mx, My, Mz = 3 * nx + 1, 3 * ny + 1, 3 * nz + 1
-    I = 3 * ex[:, None] + li[None, :]
-    J = 3 * ey[:, None] + lj[None, :]
-    K = 3 * ez[:, None] + lk[None, :]
-    lat_id = (I + Mx * (J + My * K)).astype(np.int64)          # (nelem, 64)
-    # first touch in (element, gmsh local node) order
-    flat = lat_id.ravel()
-    uniq, first = np.unique(flat, return_index=True)
-    order = np.argsort(first, kind="stable")
-    new_of_lat = np.empty(Mx * My * Mz, dtype=np.int64)
-    new_of_lat[uniq[order]] = np.arange(uniq.size)
-    npoin = uniq.size
-    assert npoin == Mx * My * Mz
-    lat_of_new = uniq[order]
-    Ilat = lat_of_new % Mx
-    Jlat = (lat_of_new // Mx) % My
-    Klat = lat_of_new // (Mx * My)
-    node_lattice = np.stack([Ilat, Jlat, Klat], axis=1)
-    connec_orig = np.asfortranarray((new_of_lat[lat_id] + 1).astype(np.int32))
-    # periodic masters: lattice index modulo 3n
-    Im, Jm, Km = I % (3 * nx), J % (3 * ny), K % (3 * nz)
-    connec = np.asfortranarray((new_of_lat[Im + Mx * (Jm + My * Km)] + 1).astype(np.int32))
-    slave = (Ilat == 3 * nx) | (Jlat == 3 * ny) | (Klat == 3 * nz)
-    lpoin_w = (np.nonzero(~slave)[0] + 1).astype(np.int32)
-    # coordinates at FP32-rounded GLL positions
-    r = t.gll_roots
+    npoin = (3 * nx + 1) * (3 * ny + 1) * (3 * nz + 1)
+    npoin_w = 27 * nelem
+    connec_orig = np.empty((nelem, 64), np.int32, order="F")
+    connec = np.empty((nelem, 64), np.int32, order="F")
+    lpoin_w = np.empty(npoin_w, np.int32)
+    coord = np.empty((npoin, 3), order="F")
+    node_lattice = np.empty((npoin, 3), np.int32)
+    _lib.call("sod_structured_box_mesh", nx, ny, nz, float(Lx), float(Ly), float(Lz), float(curved_amp),
+              elem_lex.ctypes.data, connec_orig.ctypes.data, connec.ctypes.data, lpoin_w.ctypes.data,
+              coord.ctypes.data, node_lattice.ctypes.data)
+    return Mesh(n=(nx, ny, nz), L=(Lx, Ly, Lz), nelem=nelem, npoin=npoin, npoin_w=npoin_w,
+                connec_orig=connec_orig, connec=connec, lpoin_w=lpoin_w, coord=coord,
+                elem_lex=elem_lex, node_lattice=node_lattice)
 
-    def axis_coord(idx, nel, length):
+
+def lattice_coordinates(m: Mesh) -> np.ndarray:
+    """Unperturbed (straight-sided) coordinates of every node, (npoin,3)."""
+    r = hex64_tables().gll_roots
+    out = np.empty((m.npoin, 3))
+    for d, (nel, length) in enumerate(zip(m.n, m.L)):
+        idx = m.node_lattice[:, d].astype(np.int64)
         h = length / nel
         e = np.minimum(idx // 3, nel - 1)
-        loc = idx - 3 * e
-        return e * h + (1.0 + r[loc]) * (0.5 * h)
-
-    x = axis_coord(Ilat, nx, Lx)
-    y = axis_coord(Jlat, ny, Ly)
-    z = axis_coord(Klat, nz, Lz)
-    x0 = np.stack([x, y, z], axis=1)
-    if curved_amp != 0.0:
-        hx, hy, hz = Lx / nx, Ly / ny, Lz / nz
-        sx, sy, sz = np.sin(2 * np.pi * x / Lx), np.sin(2 * np.pi * y / Ly), np.sin(2 * np.pi * z / Lz)
-        x = x + curved_amp * hx * sy * sz
-        y = y + curved_amp * hy * sz * sx
-        z = z + curved_amp * hz * sx * sy
-    coord = np.asfortranarray(np.stack([x, y, z], axis=1))
-    return Mesh(n=(nx, ny, nz), L=(Lx, Ly, Lz), nelem=nelem, npoin=int(npoin),
-                npoin_w=int(lpoin_w.size), connec_orig=connec_orig, connec=connec,
-                lpoin_w=lpoin_w, coord=coord, elem_lex=elem_lex, node_lattice=node_lattice,
-                x0=np.asfortranarray(x0))
+        out[:, d] = e * h + (1.0 + r[idx - 3 * e]) * (0.5 * h)
+    return out

@@ -148,6 +148,32 @@ def test_full_diffusion(case):
         assert err < TOL1, (k, err)
 
 
+@pytest.mark.parametrize("with_sgs", [False, True])
+def test_full_rhs(case, with_sgs):
+    """The fused RHS equals the sum of the reference routines it replaces."""
+    name, m, g, gas, s = case
+    f = stage_fields(s, gas.gamma_gas, gas.Rgas)
+    mu = oracle.sutherland_viscosity(f["T"], s["mu_factor"])
+    rng = np.random.default_rng(11)
+    mu_e = np.asfortranarray(rng.uniform(0, 2e-3, (m.nelem, 64)))
+    mu_sgs = np.asfortranarray(rng.uniform(0, 1e-3, (m.nelem, 64)) if with_sgs else np.zeros((m.nelem, 64)))
+    cm, cq, ce = oracle.full_convec_ijk(m.connec, g["He"], g["gpvol"], T, f["u"], s["q"], s["rho"], f["p"], f["h"])
+    dm, dq, de = oracle.full_diffusion_ijk(m.connec, g["He"], g["gpvol"], T, gas.Cp, gas.Prt, s["rho"], s["rho"],
+                                           f["u"], f["T"], mu, mu_e, mu_sgs)
+    rt = oracle.generic_scalar_convec_ijk(m.connec, g["He"], g["gpvol"], T, f["feta"], f["eta"], f["u"])
+    D = Dev()
+    npn = m.npoin
+    _lib.call("sod_full_rhs", m.nelem, npn, m.npoin_w, D("c", m.connec), D("lw", m.lpoin_w), D("He", g["He"]),
+              D("gv", g["gpvol"]), D("Ml", g["Ml"]), D("r", s["rho"]), D("q", s["q"]), D("E", s["E"]),
+              D("mf", s["mu_factor"]), D("mue", mu_e), D("sgs", mu_sgs), gas.Rgas, gas.gamma_gas, gas.Cp, gas.Prt,
+              D.out("Rm", npn), D.out("Rq", 3 * npn), D.out("Re", npn), D.out("Rt", npn))
+    for k, shp, r in (("Rm", (npn,), cm + dm), ("Rq", (npn, 3), cq + dq), ("Re", (npn,), ce + de),
+                      ("Rt", (npn,), rt)):
+        err = rel_l2(D.get(k, shp), r)
+        print(f"  full_rhs {k} rel_l2={err:.3e}", flush=True)
+        assert err < TOL1, (k, err)
+
+
 def test_smart_visc_dt_ek(case):
     name, m, g, gas, s = case
     rng = np.random.default_rng(9)
@@ -191,10 +217,12 @@ def _oracle_steps(m, g, gas, s, dts):
     return s
 
 
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "refseq"])
 @pytest.mark.parametrize("nsteps,tol", [(1, TOL1), (10, TOL10)])
-def test_rk_4_main(case, nsteps, tol):
+def test_rk_4_main(case, nsteps, tol, path):
     from sod2d_refmap_amd.solver import PeriodicSolver
     name, m, g, gas, s = case
+    _lib.call("sod_set_rk4_path", path)
     sol = PeriodicSolver(m, gas, s)
     # geometry computed on the device matches the oracle's
     assert rel_l2(to_host(sol.He, (3, 3, 64, m.nelem)), g["He"]) < 1e-13
@@ -206,11 +234,14 @@ def test_rk_4_main(case, nsteps, tol):
         sol.rk_4_main(d)
     out = sol.state()
     w = m.lpoin_w - 1
-    for k in ("rho", "q", "E", "u", "pr", "Tem", "eta", "csound", "machno", "mu_fluid", "mu_e", "mue_l"):
-        a, b = out[k], ref[k]
-        err = rel_l2(a, b)
-        print(f"  rk4 x{nsteps} {k} rel_l2={err:.3e}", flush=True)
-        assert err < tol, (k, err)
+    try:
+        for k in ("rho", "q", "E", "u", "pr", "Tem", "eta", "csound", "machno", "mu_fluid", "mu_e", "mue_l"):
+            a, b = out[k], ref[k]
+            err = rel_l2(a, b)
+            print(f"  rk4[{path}] x{nsteps} {k} rel_l2={err:.3e}", flush=True)
+            assert err < tol, (k, err)
+    finally:
+        _lib.call("sod_set_rk4_path", 0)
 
 
 def test_present_table_host_pointers(gpu):
