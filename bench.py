@@ -118,20 +118,29 @@ def main():
 
     # dominant kernel and its algorithmic bytes per launch
     kernels = {}
-    for name in ("full_convec", "full_diffusion", "scalar_convec", "rk4_step"):
+    for name in ("stage_rhs", "stage_update", "smart_visc_fused", "full_convec", "full_diffusion",
+                 "scalar_convec", "rk4_step"):
         ms, cnt = kernel_ms(name)
         if cnt:
             kernels[name] = (ms, cnt)
     dom = max((k for k in kernels if k != "rk4_step"), key=lambda k: kernels[k][0])
     ms, cnt = kernels[dom]
     Ne, Nn = m.nelem, m.npoin_w
-    # per element: connec 64*4 + He 576*8 + gpvol 64*8; per node: gathered
-    # inputs and scattered outputs of the routine, 8 bytes each
-    per_node = {"full_convec": 9 + 5, "full_diffusion": 6 + 5, "scalar_convec": 7 + 1}[dom]
-    per_elem = 64 * 4 + 576 * 8 + 64 * 8 + (64 * 16 if dom == "full_diffusion" else 0)
+    # Algorithmic bytes per launch (FP64, int32 ids).  Per element, the
+    # element-node data every element kernel streams: connec 64*4 + He 576*8
+    # + gpvol 64*8 (+ mu_e 64*8 for the fused RHS and the diffusion).  Per
+    # working node: the nodal inputs gathered once and the residuals written
+    # once, 8 bytes each.
+    per_elem = {"stage_rhs": 5888, "full_diffusion": 5888}.get(dom, 5376)
+    per_node = {"stage_rhs": 5 + 1 + 6, "full_convec": 9 + 5, "full_diffusion": 6 + 5,
+                "scalar_convec": 7 + 1, "stage_update": 0, "smart_visc_fused": 0}[dom]
     traffic_alg = Ne * per_elem + Nn * 8 * per_node
     avg_s = ms / cnt / 1e3
     achieved = traffic_alg / avg_s / 1e9
+    # whole-stage algorithmic bytes (element data + node state, residual,
+    # mass, RK accumulators and next-stage state): N_e*5888 + N_n*272
+    stage_bytes = Ne * 5888 + Nn * 272
+    stage_gbs = stage_bytes / (el / stages) / 1e9
     out = {
         "metric": "GDOF/s per RK4 stage (FP64, HEX64)",
         "value": round(gdofs, 4),
@@ -151,6 +160,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "avg_launch_ms": round(ms / cnt, 4), "alg_bytes_per_launch": traffic_alg},
         "kernels_ms_per_step": {k: round(v[0] / K, 4) for k, v in kernels.items()},
+        "stage": {"alg_bytes": stage_bytes, "ms": round(el / stages * 1e3, 4),
+                  "alg_GBs": round(stage_gbs, 1), "frac_of_hbm_peak": round(stage_gbs / HBM_PEAK_GBS, 4)},
         "ek_final": ek,
     }
     if not args.no_cpu:

@@ -39,6 +39,17 @@ __device__ __forceinline__ int swz(int i, int j, int k) {
     return 16 * k + ((i + k) & 3) + 4 * ((j + k) & 3);
 }
 
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -393,6 +404,131 @@ __global__ void k_zero_int(int* p) {
     if (threadIdx.x == 0) *p = 0;
 }
 
+// Tiled transpose of up to two element-node arrays (reference layout
+// (nelem,64), element fastest) into lane layout [e][lane]; 16 elements per
+// block so that both the reads and the writes are contiguous.  `nonzero` is
+// set when the second array holds a non-zero value.
+__global__ __launch_bounds__(256) void k_to_lane_tiled(int nelem, const double* __restrict__ srcA,
+                                                       double* __restrict__ dstA, const double* __restrict__ srcB,
+                                                       double* __restrict__ dstB, int* __restrict__ nonzero) {
+    __shared__ double sA[16][65], sB[16][65];
+    __shared__ int s_l2n[64];
+    if (threadIdx.x < 64) s_l2n[threadIdx.x] = c_tab.lane2node[threadIdx.x];
+    const int e0 = blockIdx.x * 16;
+    const size_t ne = (size_t)nelem;
+    int nz = 0;
+    for (int idx = threadIdx.x; idx < 1024; idx += 256) {
+        const int a = idx >> 4, el = idx & 15, e = e0 + el;
+        if (e < nelem) {
+            sA[el][a] = srcA[(size_t)e + ne * a];
+            if (srcB) {
+                const double v = srcB[(size_t)e + ne * a];
+                sB[el][a] = v;
+                nz |= v != 0.0;
+            }
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 1024; idx += 256) {
+        const int el = idx >> 6, l = idx & 63, e = e0 + el;
+        if (e < nelem) {
+            dstA[(size_t)e * 64 + l] = sA[el][s_l2n[l]];
+            if (srcB) dstB[(size_t)e * 64 + l] = sB[el][s_l2n[l]];
+        }
+    }
+    if (nz && nonzero) *nonzero = 1;
+}
+
+// smart_visc_spectral element loop (mod_entropy_viscosity.f90:68-122) on
+// lane-layout mesh data: a block takes 16 consecutive elements, one wavefront
+// per element at a time; results are staged in LDS and written to the
+// reference-layout mu_e(nelem,64) / mue_l(nelem,64) with contiguous stores.
+__global__ __launch_bounds__(256) void k_smart_visc_lane(
+    int nelem, int npoin, const int* __restrict__ conn_l, const double* __restrict__ gpv_l,
+    const double* __restrict__ helem_l, const double* __restrict__ helem, const double* __restrict__ Reta,
+    double gamma_gas, const double* __restrict__ rho, const double* __restrict__ u,
+    const double* __restrict__ csound, const double* __restrict__ Tem, const double* __restrict__ d_norm,
+    int normalise, double Rgas, double ce, double cmax, double cglob, double* __restrict__ mu_e,
+    double* __restrict__ mue_l) {
+    __shared__ double s_ml[16][65], s_me[16][65];
+    __shared__ int s_l2n[64];
+    __shared__ double s_w[64];
+    if (threadIdx.x < 64) {
+        const int a = c_tab.lane2node[threadIdx.x];
+        s_l2n[threadIdx.x] = a;
+        s_w[threadIdx.x] = c_tab.wgp[a];
+    }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int e0 = blockIdx.x * 16;
+    const size_t np = (size_t)npoin, ne = (size_t)nelem;
+    // helem_l(nelem,64) of the block's elements, read contiguously into s_me
+    for (int idx = threadIdx.x; idx < 1024; idx += 256) {
+        const int a2 = idx >> 4, el = idx & 15, e = e0 + el;
+        if (e < nelem) s_me[el][a2] = helem_l[(size_t)e + ne * a2];
+    }
+    __syncthreads();
+    const double norm = normalise ? d_norm[0] : 1.0;
+    const int a = s_l2n[l];
+    for (int r = 0; r < 4; ++r) {
+        const int el = w + 4 * r, e = e0 + el;
+        if (e >= nelem) break;   // uniform per wavefront
+        const size_t eb = (size_t)e * 64 + l;
+        const int n = conn_l[eb];
+        const double J = gpv_l[eb] / s_w[l];
+        const double u0 = u[n], u1 = u[n + np], u2 = u[n + 2 * np];
+        const double vmag = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+        const double minJe = wave_min(fmin(1000000.0, J));
+        const double maxJe = wave_max(fmax(0.0, J));
+        const double maxV = wave_max(vmag);
+        const double maxC = wave_max(csound[n]);
+        const double M = maxV / maxC;
+        double M15 = M * M;
+        M15 = M15 * M15 * M;
+        M15 = M15 * M15 * M15;
+        const double ceM = fmax(tanh(M15 * 3.14159265358979323846), ce);
+        const double rr = minJe / maxJe;
+        double ced = fmax(1.0 - rr * rr, ce);
+        ced = fmax(ced, ceM);
+        const double aux3 = sqrt(Rgas * gamma_gas * Tem[n]);
+        const double betae = wave_max((rho[n] * helem[e]) * (cmax / 3.0) * (vmag + aux3));
+        const double R1 = rho[n] * fabs(Reta[n]) / norm;
+        const double hl = s_me[el][a];
+        const double Ve = ced * R1 * (hl * hl);
+        s_ml[el][a] = cglob * fmin(Ve, betae);
+    }
+    __syncthreads();
+    // 27-point (1/4,1/2,1/4)^3 filter over geometric positions; the index
+    // table convertIJK(0:5) = {3,1,3,4,2,3} (mod_filters.f90:42-51) maps a
+    // geometric position to a sod2d 1-D index, with its end entries as built
+    const int conv[6] = {3, 1, 3, 4, 2, 3};
+    const int gi = (l & 3) + 1, gj = ((l >> 2) & 3) + 1, gk = (l >> 4) + 1;
+    const double w3[3] = {0.25, 0.5, 0.25};
+    const int so = s_l2n[(conv[gi] - 1) + 4 * (conv[gj] - 1) + 16 * (conv[gk] - 1)];
+    for (int r = 0; r < 4; ++r) {
+        const int el = w + 4 * r, e = e0 + el;
+        if (e >= nelem) break;
+        double acc = 0.0;
+#pragma unroll
+        for (int ll = -1; ll <= 1; ++ll)
+#pragma unroll
+            for (int mm = -1; mm <= 1; ++mm)
+#pragma unroll
+                for (int nn = -1; nn <= 1; ++nn) {
+                    const int b = s_l2n[(conv[gi + ll] - 1) + 4 * (conv[gj + mm] - 1) + 16 * (conv[gk + nn] - 1)];
+                    acc = acc + w3[ll + 1] * w3[mm + 1] * w3[nn + 1] * s_ml[el][b];
+                }
+        s_me[el][so] = acc;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 1024; idx += 256) {
+        const int a2 = idx >> 4, el = idx & 15, e = e0 + el;
+        if (e < nelem) {
+            mue_l[(size_t)e + ne * a2] = s_ml[el][a2];
+            mu_e[(size_t)e + ne * a2] = s_me[el][a2];
+        }
+    }
+}
+
 // Position-weighted fingerprint of sampled mesh data (connec, He, gpvol,
 // lpoin_w, Ml) so that a prepared mesh is rebuilt when a caller reuses the
 // same addresses for different contents.
@@ -516,11 +652,8 @@ static int prepare(int nelem, int npoin, int npoin_w, const int* connec, const i
     return SOD_OK;
 }
 
-void launch_smart_visc_fused(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
-                             const double* Reta, const double* gpvol, double gamma_gas, const double* rho,
-                             const double* u, const double* csound, const double* Tem,
-                             const double* helem_l, const double* helem, double* mu_e, double* mue_l,
-                             const double* eta, const double* partial, int npartial, double* work);
+void launch_eta_norm(int npoin_w, const int* lpoin_w, const double* eta, const double* partial, int npartial,
+                     double* work);
 
 int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
                    const double* He, const double* gpvol, const double* Ml, const double* helem,
@@ -536,10 +669,9 @@ int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const i
     // element-node viscosities in lane layout; mu_sgs enters only if non-zero
     int has_sgs = 0;
     if (nelem > 0) {
-        const dim3 g(grid_for((long)nelem * 64, BLK));
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, Pm.flag);
-        hipLaunchKernelGGL(k_elemnode_to_lane, g, dim3(BLK), 0, s, nelem, mu_e, Pm.mue, (int*)nullptr);
-        hipLaunchKernelGGL(k_elemnode_to_lane, g, dim3(BLK), 0, s, nelem, mu_sgs, Pm.musgs, Pm.flag);
+        hipLaunchKernelGGL(k_to_lane_tiled, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, mu_e, Pm.mue, mu_sgs,
+                           Pm.musgs, Pm.flag);
         SOD_HIP(hipMemcpyAsync(&has_sgs, Pm.flag, sizeof(int), hipMemcpyDeviceToHost, s));
         SOD_HIP(hipStreamSynchronize(s));
     }
@@ -597,9 +729,13 @@ int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const i
     // entropy viscosity for the next step (time_integ.f90:492-493)
     {
         SOD_PROF("smart_visc_fused");
-        launch_smart_visc_fused(nelem, npoin, npoin_w, connec, lpoin_w, st.aux, gpvol, gamma_gas, rho,
-                                u, csound, Tem, helem_l, helem, mu_e, mue_l, eta, st.partial, ugrid,
-                                st.partial + RED_BLOCKS);
+        double* vwork = st.partial + RED_BLOCKS;
+        if (prm.flag_normalise_entropy == 1) launch_eta_norm(npoin_w, lpoin_w, eta, st.partial, ugrid, vwork);
+        if (nelem > 0)
+            hipLaunchKernelGGL(k_smart_visc_lane, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, npoin, Pm.conn,
+                               Pm.gpv, helem_l, helem, st.aux, gamma_gas, rho, u, csound, Tem, vwork + 1,
+                               prm.flag_normalise_entropy == 1, prm.Rgas_inf, prm.ce, prm.cmax, prm.cglob, mu_e,
+                               mue_l);
     }
     SOD_HIP(hipGetLastError());
     return SOD_OK;
@@ -624,10 +760,9 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
     SOD_HIP(hipMemsetAsync(R, 0, sizeof(double) * np * NRES, s));
     int has_sgs = 0;
     if (nelem > 0) {
-        const dim3 g(grid_for((long)nelem * 64, BLK));
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, Pm.flag);
-        hipLaunchKernelGGL(k_elemnode_to_lane, g, dim3(BLK), 0, s, nelem, mu_e, Pm.mue, (int*)nullptr);
-        hipLaunchKernelGGL(k_elemnode_to_lane, g, dim3(BLK), 0, s, nelem, mu_sgs, Pm.musgs, Pm.flag);
+        hipLaunchKernelGGL(k_to_lane_tiled, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, mu_e, Pm.mue, mu_sgs,
+                           Pm.musgs, Pm.flag);
         SOD_HIP(hipMemcpyAsync(&has_sgs, Pm.flag, sizeof(int), hipMemcpyDeviceToHost, s));
         SOD_HIP(hipStreamSynchronize(s));
         StageParams sp{gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener};

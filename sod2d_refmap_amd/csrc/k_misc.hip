@@ -2,6 +2,7 @@
 // argument lists: Sutherland viscosity, lumped-mass solve, entropy viscosity
 // (smart_visc_spectral), CFL time step, kinetic-energy average, and the
 // init-time geometry routines (Jacobians, lumped mass, element lengths).
+#include <algorithm>
 #include <vector>
 
 #include "sod2d_internal.hpp"
@@ -190,23 +191,33 @@ __global__ __launch_bounds__(256) void k_adapt_dt(int nelem, int npoin, const in
                                                   const double* __restrict__ mu_sgs,
                                                   const double* __restrict__ rho,
                                                   unsigned long long* __restrict__ out_bits) {
-    const int le = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int e = blockIdx.x * 4 + le;
-    if (e >= nelem) return;
+    // one element per lane (connec(e,a) is contiguous in e for fixed a),
+    // grid-stride; one atomic per block
+    __shared__ double s_min[4];
     const size_t np = (size_t)npoin, ne = (size_t)nelem;
-    const int n = connec[(size_t)e + ne * l] - 1;
-    double umag = fabs(u[n]);
-    umag = fmax(umag, fabs(u[n + np]));
-    umag = fmax(umag, fabs(u[n + 2 * np]));
-    const double L3 = wave_max(umag + csound[n]);
-    const double hh = helem[e] / 7.0;
-    double dt_l = cfl_conv * hh / L3;
-    if (has_diff) {
-        const double mx = wave_max((rho[n] * mu_sgs[(size_t)e + ne * l] + mu_fluid[n]) / rho[n]);
-        const double aux4 = cfl_diff * (hh * hh) / mx;
-        dt_l = fmin(dt_l, aux4);
+    double best = HUGE_VAL;
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < nelem; e += gridDim.x * 256) {
+        double L3 = 0.0, mx = 0.0;
+        for (int a = 0; a < 64; ++a) {
+            const int n = connec[(size_t)e + ne * a] - 1;
+            double umag = fabs(u[n]);
+            umag = fmax(umag, fabs(u[n + np]));
+            umag = fmax(umag, fabs(u[n + 2 * np]));
+            L3 = fmax(L3, umag + csound[n]);
+            if (has_diff) mx = fmax(mx, (rho[n] * mu_sgs[(size_t)e + ne * a] + mu_fluid[n]) / rho[n]);
+        }
+        const double hh = helem[e] / 7.0;
+        double dt_l = cfl_conv * hh / L3;
+        if (has_diff) dt_l = fmin(dt_l, cfl_diff * (hh * hh) / mx);
+        best = fmin(best, dt_l);
     }
-    if (l == 0) atomicMin(out_bits, (unsigned long long)__double_as_longlong(dt_l));
+    best = wave_min(best);
+    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = fmin(fmin(s_min[0], s_min[1]), fmin(s_min[2], s_min[3]));
+        if (b < HUGE_VAL) atomicMin(out_bits, (unsigned long long)__double_as_longlong(b));
+    }
 }
 
 // ------------------------------------------------------------- EK
@@ -376,6 +387,17 @@ void launch_smart_visc(int nelem, int npoin, int npoin_w, const int* connec, con
                            mue_l);
 }
 
+// Entropy normalisation after a fused step (mod_entropy_viscosity.f90:43-66):
+// work[1] = max |eta - mean(eta)| over working nodes, from `npartial` block
+// partial sums of eta.  `work` holds >= RED_BLOCKS + 2 doubles.
+void launch_eta_norm(int npoin_w, const int* lpoin_w, const double* eta, const double* partial, int npartial,
+                     double* work) {
+    hipStream_t s = rt().stream;
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLK), 0, s, npartial, partial, work);
+    hipLaunchKernelGGL(k_eta_partial_dev, dim3(RED_BLOCKS), dim3(BLK), 0, s, npoin_w, lpoin_w, eta, work, work + 2);
+    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(BLK), 0, s, RED_BLOCKS, work + 2, work + 1);
+}
+
 // Entropy viscosity after a fused step: the sum of eta over working nodes
 // arrives as `npartial` block partials; `work` holds >= RED_BLOCKS + 2 doubles.
 void launch_smart_visc_fused(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
@@ -409,7 +431,7 @@ void launch_adapt_dt(int nelem, int npoin, const int* connec, const double* hele
     // initial value written by a kernel so that the sequence stays capturable
     hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, s, d_dt_bits, init);
     if (nelem > 0)
-        hipLaunchKernelGGL(k_adapt_dt, dim3(grid_for(nelem, 4)), dim3(256), 0, s, nelem, npoin, connec,
+        hipLaunchKernelGGL(k_adapt_dt, dim3(std::min(1024, grid_for(nelem, 256))), dim3(256), 0, s, nelem, npoin, connec,
                            helem, u, csound, cfl_conv, has_diff, cfl_diff, mu_fluid, mu_sgs, rho,
                            reinterpret_cast<unsigned long long*>(d_dt_bits));
 }
