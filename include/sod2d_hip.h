@@ -194,13 +194,6 @@ int sod_volAvg_EK(int nelem, int npoin, const int *connec, const double *gpvol,
                   const double *Ngp, double rho0, const double *rho, const double *u,
                   double *EK);
 
-/* rk_4_main, time_integ.f90:137-191: one RK4 step of the periodic explicit
- * path.  State arrays carry the reference's time levels (rho(npoin,3),
- * u(npoin,3,3), q(npoin,3,3), pr(npoin,3), E(npoin,3), Tem(npoin,2),
- * e_int(npoin,2), eta(npoin,3)) and level pos=2 is advanced in place.
- * noBoundaries must be true, isWallModelOn false (or numBoundsWM 0) and
- * flag_les 0; the boundary-only arguments (point2elem, lnbn_nodes, nboun)
- * are accepted and unused, as the reference ignores them in this branch. */
 /* One right-hand-side evaluation at a conserved state (rho, q(npoin,3), E):
  * the stage EOS, Sutherland viscosity, entropy residual, viscous and
  * convective residuals that one RK stage of rk_4_main forms
@@ -221,6 +214,13 @@ int sod_full_rhs(int nelem, int npoin, int npoin_w, const int *connec, const int
  * order.  Both give the reference's results to the stated tolerances. */
 int sod_set_rk4_path(int path);
 
+/* rk_4_main, time_integ.f90:137-191: one RK4 step of the periodic explicit
+ * path.  State arrays carry the reference's time levels (rho(npoin,3),
+ * u(npoin,3,3), q(npoin,3,3), pr(npoin,3), E(npoin,3), Tem(npoin,2),
+ * e_int(npoin,2), eta(npoin,3)) and level pos=2 is advanced in place.
+ * noBoundaries must be true, isWallModelOn false (or numBoundsWM 0) and
+ * flag_les 0; the boundary-only arguments (point2elem, lnbn_nodes, nboun)
+ * are accepted and unused, as the reference ignores them in this branch. */
 int sod_rk_4_main(int noBoundaries, int isWallModelOn, int nelem, int nboun, int npoin,
                   int npoin_w, int numBoundsWM, const int *point2elem, const int *lnbn_nodes,
                   const double *dlxigp_ip, const double *xgp, const int *atoIJK,

@@ -59,6 +59,37 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
     return LIB
 
 
+FC = os.environ.get("SOD_FC", "/opt/rocm/bin/amdflang")
+FORTRAN = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(FORTRAN, "_build")
+DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "shim_driver.f90")
+DRIVER = os.path.join(ROOT, "tests", "fortran", "_build", "shim_driver")
+
+
+def build_fortran(verbose: bool = True) -> str:
+    """Compiles the ISO_C_BINDING module (fortran/mod_sod2d_hip.f90) to an
+    object + .mod in fortran/_build/, and links the test driver that calls
+    the library through it (tests/fortran/shim_driver.f90)."""
+    os.makedirs(FBUILD, exist_ok=True)
+    obj = os.path.join(FBUILD, "mod_sod2d_hip.o")
+    cmd = [FC, "-O2", "-fPIC", "-c", os.path.join(FORTRAN, "mod_sod2d_hip.f90"), "-module-dir", FBUILD, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if os.path.exists(DRIVER_SRC):
+        os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+        rpath = os.path.relpath(HERE, os.path.dirname(DRIVER))
+        cmd = [FC, "-O2", DRIVER_SRC, obj, "-I", FBUILD, "-module-dir", os.path.dirname(DRIVER), "-L", HERE,
+               "-lsod2d_hip", f"-Wl,-rpath,$ORIGIN/{rpath}", "-o", DRIVER]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {obj} and {DRIVER}", flush=True)
+    return obj
+
+
 if __name__ == "__main__":
     build()
+    build_fortran()
     sys.exit(0)

@@ -56,14 +56,35 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Per-stage constants; ratios of the gas constants are formed once on the
+// host so that the element kernel divides only by node values.
 struct StageParams {
-    double gamma, Rgas, Cp, Pr, c_rho, c_ener;
+    double gamma, gm1, ig1, gog1, iRgas, CpPr, Cp09, c_rho, c_ener;
 };
+
+StageParams make_stage_params(double gamma, double Rgas, double Cp, double Pr, double c_rho, double c_ener) {
+    StageParams P;
+    P.gamma = gamma;
+    P.gm1 = gamma - 1.0;
+    P.ig1 = 1.0 / (gamma - 1.0);
+    P.gog1 = gamma / (gamma - 1.0);
+    P.iRgas = 1.0 / Rgas;
+    P.CpPr = Cp / Pr;
+    P.Cp09 = Cp / 0.9;   // turbulent Prandtl number 0.9 (elem_diffu.f90)
+    P.c_rho = c_rho;
+    P.c_ener = c_ener;
+    return P;
+}
 
 enum { P_RHO, P_U0, P_U1, P_U2, P_P, P_H, P_K, P_ETA, P_T, NPRIM };
 enum { R_MASS, R_MOM0, R_MOM1, R_MOM2, R_ENER, R_ETA, NRES };
 
 // ------------------------------------------------------------ element kernel
+// Along each direction the four points of a lane's line are visited as the
+// lane's own point (offset 0, from registers) and the three others (offsets
+// 1..3, from LDS); the derivative coefficients are tabulated by (own index,
+// offset): sDn[o][d] = D[o][(o+d)&3] for the strong-form derivative and
+// sDt[o][d] = D[(o+d)&3][o] for the weak-form divergence.
 template <bool HAS_SGS>
 __global__ __launch_bounds__(64 * WPB) void k_stage_rhs(
     int nelem, int npoin, const int* __restrict__ conn, const double* __restrict__ He,
@@ -72,11 +93,17 @@ __global__ __launch_bounds__(64 * WPB) void k_stage_rhs(
     const double* __restrict__ q2, const double* __restrict__ Ein, const double* __restrict__ mufac,
     double* __restrict__ R, StageParams P) {
     __shared__ double sh[WPB][18][64];
-    // The 4x4 derivative matrix is staged in LDS and indexed flat: indexing
-    // the __constant__ 2-D table with a lane-dependent row gave wrong
-    // addresses for this kernel with the current compiler.
-    __shared__ double sD[16];
+    // The derivative matrix is copied to LDS with a flat index first:
+    // indexing the __constant__ 2-D table with a lane-dependent row gave
+    // wrong addresses for this kernel with the current compiler.
+    __shared__ double sD[16], sDn[16], sDt[16];
     if (threadIdx.x < 16) sD[threadIdx.x] = (&c_tab.D[0][0])[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        const int o = threadIdx.x >> 2, b = (o + (threadIdx.x & 3)) & 3;
+        sDn[threadIdx.x] = sD[o * 4 + b];
+        sDt[threadIdx.x] = sD[b * 4 + o];
+    }
     __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int e = blockIdx.x * WPB + w;
@@ -89,16 +116,17 @@ __global__ __launch_bounds__(64 * WPB) void k_stage_rhs(
     const int n = conn[eb + l];
     const double r = rho[n];
     const double m0 = q0[n], m1 = q1[n], m2 = q2[n], en = Ein[n], mf = mufac[n];
-    const double u0 = m0 / r, u1 = m1 / r, u2 = m2 / r;
+    const double ri = 1.0 / r;
+    const double u0 = m0 * ri, u1 = m1 * ri, u2 = m2 * ri;
     const double umag = u0 * u0 + u1 * u1 + u2 * u2;
-    const double eint = (en / r) - 0.5 * umag;
-    const double g = P.gamma;
-    const double p = r * (g - 1.0) * eint;
-    const double h = (g / (g - 1.0)) * p / r;
-    const double T = p / (r * P.Rgas);
-    const double eta = (r / (g - 1.0)) * log(p / pow(r, g));
+    const double eint = en * ri - 0.5 * umag;
+    const double p = r * P.gm1 * eint;
+    const double h = P.gog1 * p * ri;
+    const double T = p * ri * P.iRgas;
+    const double eta = (r * P.ig1) * log(p * pow(r, -P.gamma));
     const double kk = 0.5 * umag;
-    const double muf = sutherland(mf, T);
+    // Sutherland's law (mod_fluid_viscosity.f90:35-39), T^1.5 as T*sqrt(T)
+    const double muf = mf * 0.000001458 * (T * sqrt(T)) / (T + 110.4);
     double hm[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -122,6 +150,11 @@ __global__ __launch_bounds__(64 * WPB) void k_stage_rhs(
     wave_sync();
 
     // ---- directional passes ---------------------------------------------
+    // With DU_b = D * U^c_b (U^c the contravariant velocity formed with this
+    // node's He) and t_b = rho_b DU_b, the divergence of a flux a*u is the
+    // sum of a_b DU_b and that of a*rho*u the sum of a_b t_b; the advective
+    // derivative of a (of a*rho) is the sum of a_b DA (a_b sA) with
+    // DA = D * U^c(own) and sA = rho_b DA.
     double divF0 = 0, divF1 = 0, divF2 = 0, divU0 = 0, divU1 = 0, divU2 = 0;
     double divFe = 0, divFue = 0, divFk = 0, divFuk = 0, divFeta = 0, divQ = 0;
     double Ah = 0, Arh = 0, Ak = 0, Ark = 0, Aq0 = 0, Aq1 = 0, Aq2 = 0, Aeta = 0;
@@ -132,14 +165,20 @@ __global__ __launch_bounds__(64 * WPB) void k_stage_rhs(
     for (int c = 0; c < 3; ++c) {
         const double h0 = hm[0][c], h1 = hm[1][c], h2 = hm[2][c];
         const double Ua = h0 * u0 + h1 * u1 + h2 * u2;
+        const int o = own[c];
         double dp = 0, du0 = 0, du1 = 0, du2 = 0, dT = 0, dr = 0;
 #pragma unroll
-        for (int bb = 0; bb < 4; ++bb) {
-            const int sl = c == 0 ? swz(bb, j, k) : (c == 1 ? swz(i, bb, k) : swz(i, j, bb));
-            const double Dc = sD[own[c] * 4 + bb];
-            const double rb = S[P_RHO][sl], u0b = S[P_U0][sl], u1b = S[P_U1][sl], u2b = S[P_U2][sl];
-            const double pb = S[P_P][sl], hb = S[P_H][sl], kb = S[P_K][sl], etab = S[P_ETA][sl],
-                         Tb = S[P_T][sl];
+        for (int d = 0; d < 4; ++d) {
+            const double Dc = sDn[o * 4 + d];
+            double rb, u0b, u1b, u2b, pb, hb, kb, etab, Tb;
+            if (d == 0) {
+                rb = r; u0b = u0; u1b = u1; u2b = u2; pb = p; hb = h; kb = kk; etab = eta; Tb = T;
+            } else {
+                const int bb = (o + d) & 3;
+                const int sl = c == 0 ? swz(bb, j, k) : (c == 1 ? swz(i, bb, k) : swz(i, j, bb));
+                rb = S[P_RHO][sl]; u0b = S[P_U0][sl]; u1b = S[P_U1][sl]; u2b = S[P_U2][sl];
+                pb = S[P_P][sl]; hb = S[P_H][sl]; kb = S[P_K][sl]; etab = S[P_ETA][sl]; Tb = S[P_T][sl];
+            }
             dp += Dc * pb;
             du0 += Dc * u0b;
             du1 += Dc * u1b;
@@ -147,28 +186,28 @@ __global__ __launch_bounds__(64 * WPB) void k_stage_rhs(
             dT += Dc * Tb;
             dr += Dc * rb;
             const double DU = Dc * (h0 * u0b + h1 * u1b + h2 * u2b);
-            const double rhb = rb * hb, rkb = rb * kb;
-            const double q0b = rb * u0b, q1b = rb * u1b, q2b = rb * u2b;
-            divF0 += q0b * DU;
-            divF1 += q1b * DU;
-            divF2 += q2b * DU;
+            const double t = rb * DU;
+            divF0 += u0b * t;
+            divF1 += u1b * t;
+            divF2 += u2b * t;
             divU0 += u0b * DU;
             divU1 += u1b * DU;
             divU2 += u2b * DU;
-            divFe += rhb * DU;
+            divFe += hb * t;
             divFue += hb * DU;
-            divFk += rkb * DU;
+            divFk += kb * t;
             divFuk += kb * DU;
             divFeta += etab * DU;
-            divQ += rb * DU;
+            divQ += t;
             const double DA = Dc * Ua;
+            const double sA = rb * DA;
             Ah += DA * hb;
-            Arh += DA * rhb;
+            Arh += sA * hb;
             Ak += DA * kb;
-            Ark += DA * rkb;
-            Aq0 += DA * q0b;
-            Aq1 += DA * q1b;
-            Aq2 += DA * q2b;
+            Ark += sA * kb;
+            Aq0 += sA * u0b;
+            Aq1 += sA * u1b;
+            Aq2 += sA * u2b;
             Aeta += DA * etab;
         }
 #pragma unroll
@@ -219,41 +258,52 @@ __global__ __launch_bounds__(64 * WPB) void k_stage_rhs(
     for (int a = 0; a < 3; ++a) tauU[a] = tau[a][0] * u0 + tau[a][1] * u1 + tau[a][2] * u2;
     // contravariant fluxes G_c = gpvol * sum_d He(d,c) F_d for the six vector
     // fluxes grad(rho), tau rows 0..2, tau.u, grad(T)
+    double hg[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) hg[d][c] = gv * hm[d][c];
+    double G[6][3];
     wave_sync();   // all lanes are done reading the nodal values
     {
         const double* F[6] = {gR, tau[0], tau[1], tau[2], tauU, gT};
 #pragma unroll
         for (int f = 0; f < 6; ++f)
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                S[3 * f + c][me] = gv * (hm[0][c] * F[f][0] + hm[1][c] * F[f][1] + hm[2][c] * F[f][2]);
+            for (int c = 0; c < 3; ++c) {
+                G[f][c] = hg[0][c] * F[f][0] + hg[1][c] * F[f][1] + hg[2][c] * F[f][2];
+                S[3 * f + c][me] = G[f][c];
+            }
     }
     wave_sync();
     // ---- weak divergence at this test node (elem_diffu.f90:142-194) ------
-    double dDr = 0, dDm0 = 0, dDm1 = 0, dDm2 = 0, dDtu = 0, dDT = 0;
+    double dD[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
+        const int o = own[c];
 #pragma unroll
-        for (int bb = 0; bb < 4; ++bb) {
-            const int sl = c == 0 ? swz(bb, j, k) : (c == 1 ? swz(i, bb, k) : swz(i, j, bb));
-            const double Dt = sD[bb * 4 + own[c]];   // l'_own(xi_q)
-            dDr += Dt * S[0 + c][sl];
-            dDm0 += Dt * S[3 + c][sl];
-            dDm1 += Dt * S[6 + c][sl];
-            dDm2 += Dt * S[9 + c][sl];
-            dDtu += Dt * S[12 + c][sl];
-            dDT += Dt * S[15 + c][sl];
+        for (int d = 0; d < 4; ++d) {
+            const double Dt = sDt[o * 4 + d];   // l'_own(xi_q)
+            if (d == 0) {
+#pragma unroll
+                for (int f = 0; f < 6; ++f) dD[f] += Dt * G[f][c];
+            } else {
+                const int bb = (o + d) & 3;
+                const int sl = c == 0 ? swz(bb, j, k) : (c == 1 ? swz(i, bb, k) : swz(i, j, bb));
+#pragma unroll
+                for (int f = 0; f < 6; ++f) dD[f] += Dt * S[3 * f + c][sl];
+            }
         }
     }
-    const double nu_e = P.c_rho * mu_e / r;
-    const double kappa = muf * P.Cp / P.Pr + P.c_ener * mu_e * P.Cp / P.Pr + P.Cp * r * mu_s / 0.9;
+    const double nu_e = P.c_rho * mu_e * ri;
+    const double kappa = muf * P.CpPr + P.c_ener * mu_e * P.CpPr + P.Cp09 * r * mu_s;
 
     double* Rn = R + n;
-    unsafeAtomicAdd(Rn + R_MASS * np, gv * Rm + nu_e * dDr);
-    unsafeAtomicAdd(Rn + R_MOM0 * np, gv * Rq[0] + dDm0);
-    unsafeAtomicAdd(Rn + R_MOM1 * np, gv * Rq[1] + dDm1);
-    unsafeAtomicAdd(Rn + R_MOM2 * np, gv * Rq[2] + dDm2);
-    unsafeAtomicAdd(Rn + R_ENER * np, gv * Re + dDtu + kappa * dDT);
+    unsafeAtomicAdd(Rn + R_MASS * np, gv * Rm + nu_e * dD[0]);
+    unsafeAtomicAdd(Rn + R_MOM0 * np, gv * Rq[0] + dD[1]);
+    unsafeAtomicAdd(Rn + R_MOM1 * np, gv * Rq[1] + dD[2]);
+    unsafeAtomicAdd(Rn + R_MOM2 * np, gv * Rq[2] + dD[3]);
+    unsafeAtomicAdd(Rn + R_ENER * np, gv * Re + dD[4] + kappa * dD[5]);
     unsafeAtomicAdd(Rn + R_ETA * np, gv * Reta);
 }
 
@@ -579,14 +629,14 @@ struct Prepared {
     int* flag = nullptr;
     size_t work_doubles = 0;
     void release() {
-        hipFree(conn);
-        hipFree(He);
-        hipFree(gpv);
-        hipFree(mue);
-        hipFree(musgs);
-        hipFree(slaves);
-        hipFree(work);
-        hipFree(flag);
+        (void)hipFree(conn);
+        (void)hipFree(He);
+        (void)hipFree(gpv);
+        (void)hipFree(mue);
+        (void)hipFree(musgs);
+        (void)hipFree(slaves);
+        (void)hipFree(work);
+        (void)hipFree(flag);
         *this = Prepared();
     }
 };
@@ -695,7 +745,7 @@ int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const i
     st.R = Pm.work + 5 * np;
     st.Rsum = Pm.work + 11 * np;
     st.partial = Pm.work + 17 * np;
-    StageParams sp{gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener};
+    const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
     const int ugrid = std::min(RED_BLOCKS, std::max(1, grid_for(npoin_w, BLK)));
     for (int stage = 0; stage < 4; ++stage) {
         const double *ir, *i0, *i1, *i2, *iE;
@@ -765,7 +815,7 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
                            Pm.musgs, Pm.flag);
         SOD_HIP(hipMemcpyAsync(&has_sgs, Pm.flag, sizeof(int), hipMemcpyDeviceToHost, s));
         SOD_HIP(hipStreamSynchronize(s));
-        StageParams sp{gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener};
+        const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
         if (has_sgs)
             hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, npoin,
                                Pm.conn, Pm.He, Pm.gpv, Pm.mue, Pm.musgs, rho, q, q + np, q + 2 * np, E, mu_factor,

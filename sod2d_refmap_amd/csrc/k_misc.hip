@@ -191,32 +191,54 @@ __global__ __launch_bounds__(256) void k_adapt_dt(int nelem, int npoin, const in
                                                   const double* __restrict__ mu_sgs,
                                                   const double* __restrict__ rho,
                                                   unsigned long long* __restrict__ out_bits) {
-    // one element per lane (connec(e,a) is contiguous in e for fixed a),
-    // grid-stride; one atomic per block
-    __shared__ double s_min[4];
+    // A block takes tiles of 16 consecutive elements: thread (el, a0) reads
+    // connec(e, a0 + 16r), r = 0..3, so each group of 16 lanes reads one
+    // contiguous run of connec and every lane has four independent gathers in
+    // flight.  Maxima over an element's 64 nodes are exact in any order.
+    __shared__ double s_L[4][16], s_M[4][16];
     const size_t np = (size_t)npoin, ne = (size_t)nelem;
+    const int el = threadIdx.x & 15, a0 = threadIdx.x >> 4;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ntiles = (nelem + 15) / 16;
     double best = HUGE_VAL;
-    for (int e = blockIdx.x * 256 + threadIdx.x; e < nelem; e += gridDim.x * 256) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int e = t * 16 + el;
         double L3 = 0.0, mx = 0.0;
-        for (int a = 0; a < 64; ++a) {
-            const int n = connec[(size_t)e + ne * a] - 1;
-            double umag = fabs(u[n]);
-            umag = fmax(umag, fabs(u[n + np]));
-            umag = fmax(umag, fabs(u[n + 2 * np]));
-            L3 = fmax(L3, umag + csound[n]);
-            if (has_diff) mx = fmax(mx, (rho[n] * mu_sgs[(size_t)e + ne * a] + mu_fluid[n]) / rho[n]);
+        if (e < nelem) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const size_t ea = (size_t)e + ne * (a0 + 16 * r);
+                const int n = connec[ea] - 1;
+                double umag = fabs(u[n]);
+                umag = fmax(umag, fabs(u[n + np]));
+                umag = fmax(umag, fabs(u[n + 2 * np]));
+                L3 = fmax(L3, umag + csound[n]);
+                if (has_diff) mx = fmax(mx, (rho[n] * mu_sgs[ea] + mu_fluid[n]) / rho[n]);
+            }
         }
-        const double hh = helem[e] / 7.0;
-        double dt_l = cfl_conv * hh / L3;
-        if (has_diff) dt_l = fmin(dt_l, cfl_diff * (hh * hh) / mx);
-        best = fmin(best, dt_l);
+        L3 = fmax(L3, __shfl_xor(L3, 16));
+        L3 = fmax(L3, __shfl_xor(L3, 32));
+        mx = fmax(mx, __shfl_xor(mx, 16));
+        mx = fmax(mx, __shfl_xor(mx, 32));
+        if (lane < 16) {
+            s_L[w][lane] = L3;
+            s_M[w][lane] = mx;
+        }
+        __syncthreads();
+        if (threadIdx.x < 16 && e < nelem) {
+            L3 = fmax(fmax(s_L[0][el], s_L[1][el]), fmax(s_L[2][el], s_L[3][el]));
+            mx = fmax(fmax(s_M[0][el], s_M[1][el]), fmax(s_M[2][el], s_M[3][el]));
+            const double hh = helem[e] / 7.0;
+            double dt_l = cfl_conv * hh / L3;
+            if (has_diff) dt_l = fmin(dt_l, cfl_diff * (hh * hh) / mx);
+            best = fmin(best, dt_l);
+        }
+        __syncthreads();
     }
-    best = wave_min(best);
-    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = best;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double b = fmin(fmin(s_min[0], s_min[1]), fmin(s_min[2], s_min[3]));
-        if (b < HUGE_VAL) atomicMin(out_bits, (unsigned long long)__double_as_longlong(b));
+    if (threadIdx.x < 64) {
+        best = wave_min(best);
+        if (threadIdx.x == 0 && best < HUGE_VAL)
+            atomicMin(out_bits, (unsigned long long)__double_as_longlong(best));
     }
 }
 
@@ -431,7 +453,7 @@ void launch_adapt_dt(int nelem, int npoin, const int* connec, const double* hele
     // initial value written by a kernel so that the sequence stays capturable
     hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, s, d_dt_bits, init);
     if (nelem > 0)
-        hipLaunchKernelGGL(k_adapt_dt, dim3(std::min(1024, grid_for(nelem, 256))), dim3(256), 0, s, nelem, npoin, connec,
+        hipLaunchKernelGGL(k_adapt_dt, dim3(std::min(2048, grid_for(nelem, 16))), dim3(256), 0, s, nelem, npoin, connec,
                            helem, u, csound, cfl_conv, has_diff, cfl_diff, mu_fluid, mu_sgs, rho,
                            reinterpret_cast<unsigned long long*>(d_dt_bits));
 }
