@@ -235,6 +235,45 @@ int sod_rk_4_main(int noBoundaries, int isWallModelOn, int nelem, int nboun, int
                   double *ax2, double *ax3, const int *lpoin_w, double *mu_fluid,
                   const double *mu_factor, double *mue_l);
 
+/* ------------------------------------------------ whole-step driver ---- */
+
+/* The periodic explicit branch of the reference's time loop,
+ * CFDSolverBase_evalTimeIteration (CFDSolverBase.f90:1501-1694, called
+ * through CFDSolverPeriodic.f90:38-55), run on the device: per step
+ *   [level 1 <- level 2 (:1532-1539)], rk_4_main, [level 3 <- level 1
+ *   (:1552-1557)], time += dt, volAvg_EK on log steps (:1579-1581),
+ *   evalDt = adapt_dt_cfl (:1132-1141, with the diffusive limit when
+ *   flag_real_diff == 1),
+ * with dt, time, the step counter and the EK history kept in device memory.
+ * One step is captured as a HIP graph on the first run (and again when the
+ * run's settings change) and replayed; no host round trip happens between
+ * steps.  The arrays are the rk_4_main arrays with their time levels
+ * (rho(npoin,3), u(npoin,3,3), ...); the loop keeps their addresses, which
+ * must stay valid until sod_time_loop_destroy.  mu_e and mu_sgs are re-read
+ * from the caller's arrays at the start of every run.
+ * No single reference routine corresponds to the create/destroy pair; it
+ * holds what the reference keeps in module variables and OpenACC data. */
+typedef struct sod_time_loop sod_time_loop;
+int sod_time_loop_create(sod_time_loop **loop, int nelem, int npoin, int npoin_w, const int *connec,
+                         const int *lpoin_w, const double *He, const double *gpvol, const double *Ml,
+                         const double *helem, const double *helem_l, const double *mu_factor, double Rgas,
+                         double gamma_gas, double Cp, double Prt, double *rho, double *u, double *q,
+                         double *pr, double *E, double *Tem, double *csound, double *machno,
+                         double *e_int, double *eta, double *mu_e, const double *mu_sgs,
+                         double *mu_fluid, double *mue_l);
+/* Steps istep0 .. istep0+nsteps-1.  dt and time are HOST scalars (inout):
+ * dt is the step size of the first step (the reference's this%dt) and, on
+ * return, the one adapt_dt_cfl computed after the last step (fixed_dt != 0
+ * keeps dt fixed and skips adapt_dt_cfl).  A step istep is a log step when
+ * log_every > 0, istep >= log_first and (istep - log_first) % log_every == 0
+ * (save_logFile_next, CFDSolverBase.f90:989-1010,1655-1657); EK (HOST,
+ * nEK >= number of log steps) receives volAvg_EK(rho0, rho(:,2), u(:,:,2))
+ * of each.  keep_levels != 0 performs the level copies. */
+int sod_time_loop_run(sod_time_loop *loop, int istep0, int nsteps, double cfl_conv, double cfl_diff,
+                      int fixed_dt, double *dt, double *time, int log_first, int log_every, double rho0,
+                      double *EK, int nEK, int keep_levels);
+int sod_time_loop_destroy(sod_time_loop *loop);
+
 #ifdef __cplusplus
 }
 #endif

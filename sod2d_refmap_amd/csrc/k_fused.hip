@@ -329,10 +329,11 @@ constexpr int RED_BLOCKS = 1024;
 // next stage state (s < 3) or final update + EOS (s == 3).
 template <bool FINAL>
 __global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, const int* __restrict__ lpoin_w,
-                                                      int s, double dt, double gamma, double Rgas,
-                                                      StepState st) {
+                                                      int s, const double* __restrict__ d_dt, double gamma,
+                                                      double Rgas, StepState st) {
     __shared__ double s_red[BLK / 64];
     const size_t np = (size_t)npoin;
+    const double dt = *d_dt;
     double eta_acc = 0.0;
     for (int it = blockIdx.x * BLK + threadIdx.x; it < npoin_w; it += gridDim.x * BLK) {
         const int n = lpoin_w[it] - 1;
@@ -421,19 +422,6 @@ __global__ __launch_bounds__(BLK) void k_slave_sutherland(int nslave, const int*
     mu_fluid[n] = sutherland(mu_factor[n], Tem[n]);
 }
 
-// reference-layout element-node array -> lane layout; flags nonzero values
-__global__ __launch_bounds__(BLK) void k_elemnode_to_lane(int nelem, const double* __restrict__ src,
-                                                          double* __restrict__ dst,
-                                                          int* __restrict__ nonzero) {
-    const long t = (long)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (long)nelem * 64) return;
-    const int e = (int)(t >> 6), l = (int)(t & 63);
-    const int a = c_tab.lane2node[l];
-    const double v = src[(size_t)e + (size_t)nelem * a];
-    dst[t] = v;
-    if (nonzero && v != 0.0) *nonzero = 1;
-}
-
 // mesh preparation: connec (ref layout, 1-based) -> lane layout (0-based),
 // He and gpvol -> lane layout
 __global__ __launch_bounds__(BLK) void k_prepare_mesh(int nelem, const int* __restrict__ connec,
@@ -452,6 +440,10 @@ __global__ __launch_bounds__(BLK) void k_prepare_mesh(int nelem, const int* __re
 
 __global__ void k_zero_int(int* p) {
     if (threadIdx.x == 0) *p = 0;
+}
+
+__global__ void k_set_double_f(double* p, double v) {
+    if (threadIdx.x == 0) *p = v;
 }
 
 // Tiled transpose of up to two element-node arrays (reference layout
@@ -499,7 +491,7 @@ __global__ __launch_bounds__(256) void k_smart_visc_lane(
     double gamma_gas, const double* __restrict__ rho, const double* __restrict__ u,
     const double* __restrict__ csound, const double* __restrict__ Tem, const double* __restrict__ d_norm,
     int normalise, double Rgas, double ce, double cmax, double cglob, double* __restrict__ mu_e,
-    double* __restrict__ mue_l) {
+    double* __restrict__ mue_l, double* __restrict__ mue_lane) {
     __shared__ double s_ml[16][65], s_me[16][65];
     __shared__ int s_l2n[64];
     __shared__ double s_w[64];
@@ -577,6 +569,12 @@ __global__ __launch_bounds__(256) void k_smart_visc_lane(
             mu_e[(size_t)e + ne * a2] = s_me[el][a2];
         }
     }
+    if (mue_lane) {
+        for (int idx = threadIdx.x; idx < 1024; idx += 256) {
+            const int el = idx >> 6, ln = idx & 63, e = e0 + el;
+            if (e < nelem) mue_lane[(size_t)e * 64 + ln] = s_me[el][s_l2n[ln]];
+        }
+    }
 }
 
 // Position-weighted fingerprint of sampled mesh data (connec, He, gpvol,
@@ -615,6 +613,11 @@ __global__ __launch_bounds__(256) void k_fingerprint(int nelem, int npoin_w, con
 }  // namespace
 
 // ------------------------------------------------------------ prepared mesh
+// Lane-layout copies of the static mesh data, the slave-node list and the
+// step's work arrays, built once per mesh.  The drop-in entry points share
+// one instance (rebuilt when the arguments' addresses or sampled contents
+// change); each whole-step driver owns its own, so that a captured graph
+// never refers to memory another call may free.
 struct Prepared {
     const void* key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int nelem = -1, npoin = -1, npoin_w = -1;
@@ -627,6 +630,9 @@ struct Prepared {
     int nslave = 0;
     double* work = nullptr;   // aux(5) + R(6) + Rsum(6) per npoin, + partials
     int* flag = nullptr;
+    double* dscal = nullptr;  // device scalars: [0] dt of the step
+    double* fpbuf = nullptr;
+    double fp[4] = {0, 0, 0, 0};
     size_t work_doubles = 0;
     void release() {
         (void)hipFree(conn);
@@ -637,6 +643,8 @@ struct Prepared {
         (void)hipFree(slaves);
         (void)hipFree(work);
         (void)hipFree(flag);
+        (void)hipFree(dscal);
+        (void)hipFree(fpbuf);
         *this = Prepared();
     }
 };
@@ -645,25 +653,31 @@ static Prepared g_prep;
 
 void fused_release() { g_prep.release(); }
 
-static double* g_fpbuf = nullptr;
-static double g_fp[4] = {0, 0, 0, 0};
+// connectivity ids must lie in 1..npoin (the kernels gather and scatter
+// through them without further checks)
+__global__ __launch_bounds__(256) void k_check_conn(long n, const int* __restrict__ conn_l, int npoin,
+                                                    int* __restrict__ bad) {
+    int b = 0;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const int v = conn_l[t];
+        b |= (v < 0) | (v >= npoin);
+    }
+    if (b) *bad = 1;
+}
 
-static int prepare(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
+static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
                    const double* He, const double* gpvol, const double* Ml) {
-    Prepared& P = g_prep;
     const void* key[6] = {connec, lpoin_w, He, gpvol, Ml, nullptr};
     bool same = P.nelem == nelem && P.npoin == npoin && P.npoin_w == npoin_w;
     for (int t = 0; t < 5 && same; ++t) same = P.key[t] == key[t];
     double fp[4] = {0, 0, 0, 0};
-    if (nelem > 0 && npoin_w > 0) {
-        if (!g_fpbuf) SOD_HIP(hipMalloc(&g_fpbuf, 4 * sizeof(double)));
+    if (same && nelem > 0 && npoin_w > 0) {
         hipLaunchKernelGGL(k_fingerprint, dim3(1), dim3(256), 0, rt().stream, nelem, npoin_w, connec, He, gpvol,
-                           lpoin_w, Ml, g_fpbuf);
-        SOD_HIP(hipMemcpyAsync(fp, g_fpbuf, sizeof(fp), hipMemcpyDeviceToHost, rt().stream));
+                           lpoin_w, Ml, P.fpbuf);
+        SOD_HIP(hipMemcpyAsync(fp, P.fpbuf, sizeof(fp), hipMemcpyDeviceToHost, rt().stream));
         SOD_HIP(hipStreamSynchronize(rt().stream));
+        if (std::memcmp(fp, P.fp, sizeof(fp)) == 0) return SOD_OK;
     }
-    if (same && std::memcmp(fp, g_fp, sizeof(fp)) == 0) return SOD_OK;
-    std::memcpy(g_fp, fp, sizeof(fp));
     SOD_HIP(hipStreamSynchronize(rt().stream));
     P.release();
     const size_t ne = (size_t)nelem, np = (size_t)npoin;
@@ -673,19 +687,35 @@ static int prepare(int nelem, int npoin, int npoin_w, const int* connec, const i
     SOD_HIP(hipMalloc(&P.mue, sizeof(double) * ne * 64 + 8));
     SOD_HIP(hipMalloc(&P.musgs, sizeof(double) * ne * 64 + 8));
     SOD_HIP(hipMalloc(&P.flag, sizeof(int) * 4));
+    SOD_HIP(hipMalloc(&P.dscal, sizeof(double) * 8));
+    SOD_HIP(hipMalloc(&P.fpbuf, sizeof(double) * 4));
     P.work_doubles = np * 17 + 2 * RED_BLOCKS + 64;
     SOD_HIP(hipMalloc(&P.work, sizeof(double) * P.work_doubles));
     SOD_HIP(hipMemsetAsync(P.work, 0, sizeof(double) * P.work_doubles, rt().stream));
-    if (nelem > 0)
+    SOD_HIP(hipMemsetAsync(P.musgs, 0, sizeof(double) * ne * 64, rt().stream));
+    SOD_HIP(hipMemsetAsync(P.flag, 0, sizeof(int) * 4, rt().stream));
+    int bad = 0;
+    if (nelem > 0) {
         hipLaunchKernelGGL(k_prepare_mesh, dim3(grid_for((long)nelem * 64, BLK)), dim3(BLK), 0, rt().stream,
                            nelem, connec, He, gpvol, P.conn, P.He, P.gpv);
+        hipLaunchKernelGGL(k_check_conn, dim3(std::min(1024, grid_for((long)nelem * 64, 256))), dim3(256), 0,
+                           rt().stream, (long)nelem * 64, P.conn, npoin, P.flag);
+        SOD_HIP(hipMemcpyAsync(&bad, P.flag, sizeof(int), hipMemcpyDeviceToHost, rt().stream));
+    }
     // slave list: nodes that are not in lpoin_w
     std::vector<int> hw(npoin_w);
     SOD_HIP(hipStreamSynchronize(rt().stream));
+    if (bad) {
+        P.release();
+        return fail(SOD_ERR_ARG, "connec holds an id outside 1..npoin");
+    }
     SOD_HIP(hipMemcpy(hw.data(), lpoin_w, sizeof(int) * npoin_w, hipMemcpyDeviceToHost));
     std::vector<char> isw(npoin, 0);
     for (int t = 0; t < npoin_w; ++t) {
-        if (hw[t] < 1 || hw[t] > npoin) return fail(SOD_ERR_ARG, "lpoin_w holds an id outside 1..npoin");
+        if (hw[t] < 1 || hw[t] > npoin) {
+            P.release();
+            return fail(SOD_ERR_ARG, "lpoin_w holds an id outside 1..npoin");
+        }
         isw[hw[t] - 1] = 1;
     }
     std::vector<int> sl;
@@ -694,6 +724,12 @@ static int prepare(int nelem, int npoin, int npoin_w, const int* connec, const i
     P.nslave = (int)sl.size();
     SOD_HIP(hipMalloc(&P.slaves, sizeof(int) * (sl.size() + 1)));
     if (!sl.empty()) SOD_HIP(hipMemcpy(P.slaves, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice));
+    if (nelem > 0 && npoin_w > 0) {
+        hipLaunchKernelGGL(k_fingerprint, dim3(1), dim3(256), 0, rt().stream, nelem, npoin_w, connec, He, gpvol,
+                           lpoin_w, Ml, P.fpbuf);
+        SOD_HIP(hipMemcpyAsync(P.fp, P.fpbuf, sizeof(P.fp), hipMemcpyDeviceToHost, rt().stream));
+        SOD_HIP(hipStreamSynchronize(rt().stream));
+    }
     SOD_HIP(hipGetLastError());
     for (int t = 0; t < 5; ++t) P.key[t] = key[t];
     P.nelem = nelem;
@@ -702,50 +738,65 @@ static int prepare(int nelem, int npoin, int npoin_w, const int* connec, const i
     return SOD_OK;
 }
 
+// mu_e (and mu_sgs) from the caller's layout into P's lane layout; returns
+// whether mu_sgs holds a non-zero value (host synchronisation).
+static int load_viscosities(Prepared& P, int nelem, const double* mu_e, const double* mu_sgs, int* has_sgs) {
+    hipStream_t s = rt().stream;
+    *has_sgs = 0;
+    if (nelem <= 0) return SOD_OK;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, P.flag);
+    hipLaunchKernelGGL(k_to_lane_tiled, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, mu_e, P.mue, mu_sgs,
+                       P.musgs, P.flag);
+    SOD_HIP(hipMemcpyAsync(has_sgs, P.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    SOD_HIP(hipStreamSynchronize(s));
+    return SOD_OK;
+}
+
 void launch_eta_norm(int npoin_w, const int* lpoin_w, const double* eta, const double* partial, int npartial,
                      double* work);
 
-int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
-                   const double* He, const double* gpvol, const double* Ml, const double* helem,
-                   const double* helem_l, double dt, double Rgas, double gamma_gas, double Cp, double Prt,
-                   const double* mu_factor, double* rho, double* u, double* q, double* pr, double* E,
-                   double* Tem, double* e_int, double* eta, double* csound, double* machno,
-                   double* mu_fluid, double* mu_e, const double* mu_sgs, double* mue_l) {
-    SOD_TRY(prepare(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
-    Prepared& Pm = g_prep;
+// Level-2 views of the rk_4_main arguments (device addresses).
+struct FusedArgs {
+    int nelem, npoin, npoin_w;
+    const int *connec, *lpoin_w;
+    const double *He, *gpvol, *Ml, *helem, *helem_l, *mu_factor;
+    double Rgas, gamma_gas, Cp, Prt;
+    double *rho, *u, *q, *pr, *E, *Tem, *e_int, *eta, *csound, *machno, *mu_fluid, *mu_e, *mue_l;
+    const double* mu_sgs;
+};
+
+// Enqueues one fused RK4 step on the library stream without any host
+// synchronisation (so that it can be captured in a graph): mu_e / mu_sgs are
+// read from P's lane-layout copies, dt from device memory.  With
+// `mue_lane_out` the new entropy viscosity is also left in lane layout in P
+// for the next step.
+static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_dt, bool has_sgs,
+                               bool mue_lane_out) {
     hipStream_t s = rt().stream;
     const sod_numerical_params& prm = rt().params;
+    const int nelem = a.nelem, npoin = a.npoin, npoin_w = a.npoin_w;
     const size_t np = (size_t)npoin;
-    // element-node viscosities in lane layout; mu_sgs enters only if non-zero
-    int has_sgs = 0;
-    if (nelem > 0) {
-        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, Pm.flag);
-        hipLaunchKernelGGL(k_to_lane_tiled, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, mu_e, Pm.mue, mu_sgs,
-                           Pm.musgs, Pm.flag);
-        SOD_HIP(hipMemcpyAsync(&has_sgs, Pm.flag, sizeof(int), hipMemcpyDeviceToHost, s));
-        SOD_HIP(hipStreamSynchronize(s));
-    }
     StepState st;
-    st.rho = rho;
-    st.q0 = q;
-    st.q1 = q + np;
-    st.q2 = q + 2 * np;
-    st.E = E;
-    st.u = u;
-    st.pr = pr;
-    st.Tem = Tem;
-    st.e_int = e_int;
-    st.eta = eta;
-    st.csound = csound;
-    st.machno = machno;
-    st.mu_fluid = mu_fluid;
-    st.Ml = Ml;
-    st.mu_factor = mu_factor;
-    st.aux = Pm.work;
-    st.R = Pm.work + 5 * np;
-    st.Rsum = Pm.work + 11 * np;
-    st.partial = Pm.work + 17 * np;
-    const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
+    st.rho = a.rho;
+    st.q0 = a.q;
+    st.q1 = a.q + np;
+    st.q2 = a.q + 2 * np;
+    st.E = a.E;
+    st.u = a.u;
+    st.pr = a.pr;
+    st.Tem = a.Tem;
+    st.e_int = a.e_int;
+    st.eta = a.eta;
+    st.csound = a.csound;
+    st.machno = a.machno;
+    st.mu_fluid = a.mu_fluid;
+    st.Ml = a.Ml;
+    st.mu_factor = a.mu_factor;
+    st.aux = P.work;
+    st.R = P.work + 5 * np;
+    st.Rsum = P.work + 11 * np;
+    st.partial = P.work + 17 * np;
+    const StageParams sp = make_stage_params(a.gamma_gas, a.Rgas, a.Cp, a.Prt, prm.c_rho, prm.c_ener);
     const int ugrid = std::min(RED_BLOCKS, std::max(1, grid_for(npoin_w, BLK)));
     for (int stage = 0; stage < 4; ++stage) {
         const double *ir, *i0, *i1, *i2, *iE;
@@ -758,35 +809,52 @@ int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const i
             SOD_PROF("stage_rhs");
             if (has_sgs)
                 hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem,
-                                   npoin, Pm.conn, Pm.He, Pm.gpv, Pm.mue, Pm.musgs, ir, i0, i1, i2, iE,
-                                   mu_factor, st.R, sp);
+                                   npoin, P.conn, P.He, P.gpv, P.mue, P.musgs, ir, i0, i1, i2, iE, a.mu_factor,
+                                   st.R, sp);
             else
                 hipLaunchKernelGGL(k_stage_rhs<false>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem,
-                                   npoin, Pm.conn, Pm.He, Pm.gpv, Pm.mue, Pm.musgs, ir, i0, i1, i2, iE,
-                                   mu_factor, st.R, sp);
+                                   npoin, P.conn, P.He, P.gpv, P.mue, P.musgs, ir, i0, i1, i2, iE, a.mu_factor,
+                                   st.R, sp);
         }
         SOD_PROF("stage_update");
         if (stage < 3)
-            hipLaunchKernelGGL(k_stage_update<false>, dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w, lpoin_w, stage,
-                               dt, gamma_gas, Rgas, st);
+            hipLaunchKernelGGL(k_stage_update<false>, dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w, a.lpoin_w,
+                               stage, d_dt, a.gamma_gas, a.Rgas, st);
         else
-            hipLaunchKernelGGL(k_stage_update<true>, dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w, lpoin_w, stage,
-                               dt, gamma_gas, Rgas, st);
+            hipLaunchKernelGGL(k_stage_update<true>, dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w, a.lpoin_w,
+                               stage, d_dt, a.gamma_gas, a.Rgas, st);
     }
-    if (Pm.nslave > 0)
-        hipLaunchKernelGGL(k_slave_sutherland, dim3(grid_for(Pm.nslave, BLK)), dim3(BLK), 0, s, Pm.nslave,
-                           Pm.slaves, Tem, mu_factor, mu_fluid);
+    if (P.nslave > 0)
+        hipLaunchKernelGGL(k_slave_sutherland, dim3(grid_for(P.nslave, BLK)), dim3(BLK), 0, s, P.nslave,
+                           P.slaves, a.Tem, a.mu_factor, a.mu_fluid);
     // entropy viscosity for the next step (time_integ.f90:492-493)
     {
         SOD_PROF("smart_visc_fused");
         double* vwork = st.partial + RED_BLOCKS;
-        if (prm.flag_normalise_entropy == 1) launch_eta_norm(npoin_w, lpoin_w, eta, st.partial, ugrid, vwork);
+        if (prm.flag_normalise_entropy == 1) launch_eta_norm(npoin_w, a.lpoin_w, a.eta, st.partial, ugrid, vwork);
         if (nelem > 0)
-            hipLaunchKernelGGL(k_smart_visc_lane, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, npoin, Pm.conn,
-                               Pm.gpv, helem_l, helem, st.aux, gamma_gas, rho, u, csound, Tem, vwork + 1,
-                               prm.flag_normalise_entropy == 1, prm.Rgas_inf, prm.ce, prm.cmax, prm.cglob, mu_e,
-                               mue_l);
+            hipLaunchKernelGGL(k_smart_visc_lane, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, npoin, P.conn,
+                               P.gpv, a.helem_l, a.helem, st.aux, a.gamma_gas, a.rho, a.u, a.csound, a.Tem,
+                               vwork + 1, prm.flag_normalise_entropy == 1, prm.Rgas_inf, prm.ce, prm.cmax,
+                               prm.cglob, a.mu_e, a.mue_l, mue_lane_out ? P.mue : nullptr);
     }
+}
+
+int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
+                   const double* He, const double* gpvol, const double* Ml, const double* helem,
+                   const double* helem_l, double dt, double Rgas, double gamma_gas, double Cp, double Prt,
+                   const double* mu_factor, double* rho, double* u, double* q, double* pr, double* E,
+                   double* Tem, double* e_int, double* eta, double* csound, double* machno,
+                   double* mu_fluid, double* mu_e, const double* mu_sgs, double* mue_l) {
+    Prepared& P = g_prep;
+    SOD_TRY(prepare(P, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
+    int has_sgs = 0;
+    SOD_TRY(load_viscosities(P, nelem, mu_e, mu_sgs, &has_sgs));
+    const FusedArgs a{nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml, helem, helem_l, mu_factor, Rgas,
+                      gamma_gas, Cp, Prt, rho, u, q, pr, E, Tem, e_int, eta, csound, machno, mu_fluid, mu_e,
+                      mue_l, mu_sgs};
+    hipLaunchKernelGGL(k_set_double_f, dim3(1), dim3(64), 0, rt().stream, P.dscal, dt);
+    enqueue_fused_step(P, a, P.dscal, has_sgs != 0, false);
     SOD_HIP(hipGetLastError());
     return SOD_OK;
 }
@@ -799,8 +867,8 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
              const double* gpvol, const double* Ml, const double* rho, const double* q, const double* E,
              const double* mu_factor, const double* mu_e, const double* mu_sgs, double Rgas, double gamma_gas,
              double Cp, double Prt, double* Rmass, double* Rmom, double* Rener, double* Reta) {
-    SOD_TRY(prepare(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
     Prepared& Pm = g_prep;
+    SOD_TRY(prepare(Pm, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
     hipStream_t s = rt().stream;
     const sod_numerical_params& prm = rt().params;
     const size_t np = (size_t)npoin;
@@ -809,12 +877,8 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
     double* R = static_cast<double*>(buf);
     SOD_HIP(hipMemsetAsync(R, 0, sizeof(double) * np * NRES, s));
     int has_sgs = 0;
+    SOD_TRY(load_viscosities(Pm, nelem, mu_e, mu_sgs, &has_sgs));
     if (nelem > 0) {
-        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, Pm.flag);
-        hipLaunchKernelGGL(k_to_lane_tiled, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, mu_e, Pm.mue, mu_sgs,
-                           Pm.musgs, Pm.flag);
-        SOD_HIP(hipMemcpyAsync(&has_sgs, Pm.flag, sizeof(int), hipMemcpyDeviceToHost, s));
-        SOD_HIP(hipStreamSynchronize(s));
         const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
         if (has_sgs)
             hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, npoin,
@@ -831,6 +895,182 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
     SOD_HIP(hipMemcpyAsync(Reta, R + R_ETA * np, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
     SOD_HIP(hipGetLastError());
     return SOD_OK;
+}
+
+
+// ------------------------------------------------------- whole-step driver
+// Device-side pieces of the reference's time loop
+// (CFDSolverBase_evalTimeIteration, CFDSolverBase.f90:1501-1694) around
+// rk_4_main.  All of them read their scalars from device memory so that a
+// step can be captured once in a graph and replayed.
+
+constexpr int kMaxCopy = 12;
+struct CopyList {
+    const double* src[kMaxCopy];
+    double* dst[kMaxCopy];
+};
+
+// dst[y][i] = src[y][i], i < n, for the blockIdx.y-th pair
+__global__ __launch_bounds__(256) void k_copy_fields(long n, CopyList c) {
+    const double* __restrict__ s = c.src[blockIdx.y];
+    double* __restrict__ d = c.dst[blockIdx.y];
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) d[i] = s[i];
+}
+
+// loop state on the device: [0] dt, [1] time, [2] EK scratch; ints: [0]
+// istep of the step being taken, [1] EK values recorded in this run
+__global__ void k_loop_begin(int* __restrict__ ist) {
+    if (threadIdx.x == 0) ist[0] += 1;
+}
+
+__global__ void k_loop_time(double* __restrict__ sc) {
+    if (threadIdx.x == 0) sc[1] = sc[1] + sc[0];
+}
+
+__device__ __forceinline__ bool is_log_step(int istep, int first, int every) {
+    return every > 0 && istep >= first && (istep - first) % every == 0;
+}
+
+// volAvg_EK (mod_analysis.f90:10-44) on log steps only; same sums as the
+// drop-in entry point
+__global__ __launch_bounds__(256) void k_ek_partial_log(int nelem, int npoin, const int* __restrict__ connec,
+                                                        const double* __restrict__ gpvol,
+                                                        const double* __restrict__ Ngp,
+                                                        const double* __restrict__ rho,
+                                                        const double* __restrict__ u, const int* __restrict__ ist,
+                                                        int first, int every, double* __restrict__ partial) {
+    if (!is_log_step(ist[0], first, every)) return;
+    __shared__ double s_tmp[4];
+    const int le = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const size_t np = (size_t)npoin, ne = (size_t)nelem;
+    double acc = 0.0;
+    for (int e = blockIdx.x * 4 + le; e < nelem; e += gridDim.x * 4) {
+        const int n = connec[(size_t)e + ne * l] - 1;
+        const double u0 = u[n], u1 = u[n + np], u2 = u[n + 2 * np];
+        const double ke = 0.5 * rho[n] * (u0 * u0 + u1 * u1 + u2 * u2);
+        const double* gv = gpvol + 64 * (size_t)e;
+        for (int g = 0; g < 64; ++g) acc += gv[g] * Ngp[g + 64 * l] * ke;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (l == 0) s_tmp[le] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (s_tmp[0] + s_tmp[1]) + (s_tmp[2] + s_tmp[3]);
+}
+
+__global__ __launch_bounds__(256) void k_ek_final_log(int npartial, const double* __restrict__ partial,
+                                                      int* __restrict__ ist, int first, int every, double rho0,
+                                                      int cap, double* __restrict__ hist) {
+    if (!is_log_step(ist[0], first, every)) return;
+    __shared__ double s_tmp[4];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < npartial; i += 256) v += partial[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s_tmp[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double sum = (s_tmp[0] + s_tmp[1]) + (s_tmp[2] + s_tmp[3]);
+        const int k = ist[1];
+        // mod_analysis.f90:37 uses 3.14159 for pi
+        if (k < cap) hist[k] = sum / (rho0 * pow(2.0 * 3.14159, 3.0));
+        ist[1] = k + 1;
+    }
+}
+
+void launch_adapt_dt(int nelem, int npoin, const int* connec, const double* helem, const double* u,
+                     const double* csound, double cfl_conv, int has_diff, double cfl_diff,
+                     const double* mu_fluid, const double* mu_sgs, const double* rho, double* d_dt_bits);
+
+struct LoopCfg {
+    bool valid = false;
+    bool has_sgs = false, fixed_dt = false, keep_levels = false;
+    double cfl_conv = 0, cfl_diff = 0, rho0 = 0;
+    int log_first = 0, log_every = 0, ek_cap = 0, has_diff = 0;
+    sod_numerical_params params{};
+    bool operator==(const LoopCfg& o) const {
+        return valid == o.valid && has_sgs == o.has_sgs && fixed_dt == o.fixed_dt &&
+               keep_levels == o.keep_levels && cfl_conv == o.cfl_conv && cfl_diff == o.cfl_diff &&
+               rho0 == o.rho0 && log_first == o.log_first && log_every == o.log_every && ek_cap == o.ek_cap &&
+               has_diff == o.has_diff && std::memcmp(&params, &o.params, sizeof(params)) == 0;
+    }
+};
+
+}  // namespace sod
+
+struct sod_time_loop {
+    sod::Prepared P;
+    sod::FusedArgs a;       // level-2 views
+    double *rho, *u, *q, *pr, *E, *Tem, *e_int, *eta;   // full level arrays
+    double* Ngp = nullptr;  // device copy of the library's Ngp
+    double* sc = nullptr;   // dt, time, scratch
+    int* ist = nullptr;     // istep, EK count
+    double* ek_partial = nullptr;
+    double* ek_hist = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    sod::LoopCfg cfg;
+};
+
+namespace sod {
+
+static void loop_free_graph(sod_time_loop* L) {
+    if (L->exec) (void)hipGraphExecDestroy(L->exec);
+    if (L->graph) (void)hipGraphDestroy(L->graph);
+    L->exec = nullptr;
+    L->graph = nullptr;
+    L->cfg.valid = false;
+}
+
+// the kernels of one time-loop step, enqueued on the library stream
+static void loop_enqueue_step(sod_time_loop* L, const LoopCfg& c) {
+    hipStream_t s = rt().stream;
+    const FusedArgs& a = L->a;
+    const size_t np = (size_t)a.npoin;
+    const dim3 cgrid(std::max(1, std::min(256, grid_for((long)np, 256))), 1);
+    hipLaunchKernelGGL(k_loop_begin, dim3(1), dim3(64), 0, s, L->ist);
+    if (c.keep_levels) {
+        // level 1 <- level 2 (CFDSolverBase.f90:1532-1539)
+        CopyList cl{};
+        int k = 0;
+        auto add = [&](double* base, int ncomp, int nlev) {
+            for (int d = 0; d < ncomp; ++d) {
+                cl.src[k] = base + np * ((size_t)ncomp * 1 + d);
+                cl.dst[k] = base + np * d;
+                ++k;
+            }
+            (void)nlev;
+        };
+        add(L->rho, 1, 3); add(L->u, 3, 3); add(L->q, 3, 3); add(L->pr, 1, 3); add(L->E, 1, 3);
+        add(L->Tem, 1, 2); add(L->e_int, 1, 2); add(L->eta, 1, 3);
+        hipLaunchKernelGGL(k_copy_fields, dim3(cgrid.x, k), dim3(256), 0, s, (long)np, cl);
+    }
+    enqueue_fused_step(L->P, a, L->sc, c.has_sgs, true);
+    if (c.keep_levels) {
+        // level 3 <- level 1 (CFDSolverBase.f90:1552-1557)
+        CopyList cl{};
+        int k = 0;
+        auto add = [&](double* base, int ncomp) {
+            for (int d = 0; d < ncomp; ++d) {
+                cl.src[k] = base + np * d;
+                cl.dst[k] = base + np * ((size_t)ncomp * 2 + d);
+                ++k;
+            }
+        };
+        add(L->rho, 1); add(L->E, 1); add(L->q, 3); add(L->eta, 1); add(L->u, 3); add(L->pr, 1);
+        hipLaunchKernelGGL(k_copy_fields, dim3(cgrid.x, k), dim3(256), 0, s, (long)np, cl);
+    }
+    hipLaunchKernelGGL(k_loop_time, dim3(1), dim3(64), 0, s, L->sc);
+    if (c.log_every > 0) {
+        hipLaunchKernelGGL(k_ek_partial_log, dim3(RED_BLOCKS), dim3(256), 0, s, a.nelem, a.npoin, a.connec, a.gpvol,
+                           L->Ngp, a.rho, a.u, L->ist, c.log_first, c.log_every, L->ek_partial);
+        hipLaunchKernelGGL(k_ek_final_log, dim3(1), dim3(256), 0, s, RED_BLOCKS, L->ek_partial, L->ist,
+                           c.log_first, c.log_every, c.rho0, c.ek_cap, L->ek_hist);
+    }
+    if (!c.fixed_dt)   // evalDt (CFDSolverBase.f90:1132-1141)
+        launch_adapt_dt(a.nelem, a.npoin, a.connec, a.helem, a.u, a.csound, c.cfl_conv, c.has_diff, c.cfl_diff,
+                        a.mu_fluid, a.mu_sgs, a.rho, L->sc);
 }
 
 }  // namespace sod
@@ -865,4 +1105,158 @@ extern "C" int sod_full_rhs(int nelem, int npoin, int npoin_w, const int* connec
     SOD_TRY(full_rhs(nelem, npoin, npoin_w, dc, dl, dHe, dgv, dMl, dr, dq, dE, dmf, dmue, dsgs, Rgas, gamma_gas, Cp,
                      Prt, dRm, dRq, dRe, dRt));
     return finish_call();
+}
+
+// ------------------------------------------------- whole-step driver (C ABI)
+extern "C" int sod_time_loop_create(sod_time_loop** out, int nelem, int npoin, int npoin_w, const int* connec,
+                                    const int* lpoin_w, const double* He, const double* gpvol, const double* Ml,
+                                    const double* helem, const double* helem_l, const double* mu_factor,
+                                    double Rgas, double gamma_gas, double Cp, double Prt, double* rho, double* u,
+                                    double* q, double* pr, double* E, double* Tem, double* csound,
+                                    double* machno, double* e_int, double* eta, double* mu_e,
+                                    const double* mu_sgs, double* mu_fluid, double* mue_l) {
+    SOD_TRY(ensure_init());
+    if (out == nullptr) return fail(SOD_ERR_ARG, "time_loop_create: out is NULL");
+    *out = nullptr;
+    if (nelem < 1 || npoin < 1 || npoin_w < 1 || npoin_w > npoin)
+        return fail(SOD_ERR_ARG, "time_loop_create: inconsistent sizes");
+    const int *dc, *dl;
+    const double *dHe, *dgv, *dMl, *dh, *dhl, *dmf, *dsgs;
+    double *drho, *du, *dq, *dpr, *dE, *dT, *dcs, *dma, *dei, *deta, *dmue, *dmuf, *dmuel;
+    SOD_TRY(resolve_t(connec, "connec", &dc));
+    SOD_TRY(resolve_t(lpoin_w, "lpoin_w", &dl));
+    SOD_TRY(resolve_t(He, "He", &dHe));
+    SOD_TRY(resolve_t(gpvol, "gpvol", &dgv));
+    SOD_TRY(resolve_t(Ml, "Ml", &dMl));
+    SOD_TRY(resolve_t(helem, "helem", &dh));
+    SOD_TRY(resolve_t(helem_l, "helem_l", &dhl));
+    SOD_TRY(resolve_t(mu_factor, "mu_factor", &dmf));
+    SOD_TRY(resolve_t(mu_sgs, "mu_sgs", &dsgs));
+    SOD_TRY(resolve_w(rho, "rho", &drho));
+    SOD_TRY(resolve_w(u, "u", &du));
+    SOD_TRY(resolve_w(q, "q", &dq));
+    SOD_TRY(resolve_w(pr, "pr", &dpr));
+    SOD_TRY(resolve_w(E, "E", &dE));
+    SOD_TRY(resolve_w(Tem, "Tem", &dT));
+    SOD_TRY(resolve_w(csound, "csound", &dcs));
+    SOD_TRY(resolve_w(machno, "machno", &dma));
+    SOD_TRY(resolve_w(e_int, "e_int", &dei));
+    SOD_TRY(resolve_w(eta, "eta", &deta));
+    SOD_TRY(resolve_w(mu_e, "mu_e", &dmue));
+    SOD_TRY(resolve_w(mu_fluid, "mu_fluid", &dmuf));
+    SOD_TRY(resolve_w(mue_l, "mue_l", &dmuel));
+    auto* L = new sod_time_loop();
+    const size_t np = (size_t)npoin;
+    L->rho = drho; L->u = du; L->q = dq; L->pr = dpr; L->E = dE; L->Tem = dT; L->e_int = dei; L->eta = deta;
+    L->a = FusedArgs{nelem, npoin, npoin_w, dc, dl, dHe, dgv, dMl, dh, dhl, dmf, Rgas, gamma_gas, Cp, Prt,
+                     drho + np, du + 3 * np, dq + 3 * np, dpr + np, dE + np, dT + np, dei + np, deta + np, dcs,
+                     dma, dmuf, dmue, dmuel, dsgs};
+    int rc = prepare(L->P, nelem, npoin, npoin_w, dc, dl, dHe, dgv, dMl);
+    if (rc == SOD_OK) {
+        hipError_t e = hipMalloc(&L->Ngp, sizeof(double) * 64 * 64);
+        if (e == hipSuccess) e = hipMalloc(&L->sc, sizeof(double) * 8);
+        if (e == hipSuccess) e = hipMalloc(&L->ist, sizeof(int) * 4);
+        if (e == hipSuccess) e = hipMalloc(&L->ek_partial, sizeof(double) * RED_BLOCKS);
+        if (e == hipSuccess)
+            e = hipMemcpy(L->Ngp, host_tables().Ngp, sizeof(double) * 64 * 64, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) rc = fail(SOD_ERR_HIP, "time_loop_create: %s", hipGetErrorString(e));
+    }
+    if (rc != SOD_OK) {
+        sod_time_loop_destroy(L);
+        return rc;
+    }
+    *out = L;
+    return finish_call();
+}
+
+extern "C" int sod_time_loop_run(sod_time_loop* L, int istep0, int nsteps, double cfl_conv, double cfl_diff,
+                                 int fixed_dt, double* dt, double* time, int log_first, int log_every,
+                                 double rho0, double* EK, int nEK, int keep_levels) {
+    SOD_TRY(ensure_init());
+    if (L == nullptr || dt == nullptr || time == nullptr) return fail(SOD_ERR_ARG, "time_loop_run: NULL argument");
+    if (nsteps < 0) return fail(SOD_ERR_ARG, "time_loop_run: nsteps < 0");
+    int nlog = 0;
+    if (log_every > 0)
+        for (int i = istep0; i < istep0 + nsteps; ++i) nlog += (i >= log_first && (i - log_first) % log_every == 0);
+    if (nlog > 0 && (EK == nullptr || nEK < nlog))
+        return fail(SOD_ERR_ARG, "time_loop_run: EK must hold %d values", nlog);
+    if (nsteps == 0) return SOD_OK;
+    int has_sgs = 0;
+    SOD_TRY(load_viscosities(L->P, L->a.nelem, L->a.mu_e, L->a.mu_sgs, &has_sgs));
+    LoopCfg c;
+    c.valid = true;
+    c.has_sgs = has_sgs != 0;
+    c.fixed_dt = fixed_dt != 0;
+    c.keep_levels = keep_levels != 0;
+    c.cfl_conv = cfl_conv;
+    c.cfl_diff = cfl_diff;
+    c.rho0 = rho0;
+    c.log_first = log_first;
+    c.log_every = log_every > 0 ? log_every : 0;
+    c.params = rt().params;
+    c.has_diff = rt().params.flag_real_diff == 1;
+    c.ek_cap = std::max(L->cfg.ek_cap, std::max(1, nlog));
+    if (!L->ek_hist || c.ek_cap > L->cfg.ek_cap) {
+        loop_free_graph(L);
+        (void)hipFree(L->ek_hist);
+        L->ek_hist = nullptr;
+        SOD_HIP(hipMalloc(&L->ek_hist, sizeof(double) * c.ek_cap));
+    }
+    const double hsc[2] = {*dt, *time};
+    const int hist[2] = {istep0 - 1, 0};
+    hipStream_t saved = rt().stream;
+    SOD_HIP(hipMemcpyAsync(L->sc, hsc, sizeof(hsc), hipMemcpyHostToDevice, saved));
+    SOD_HIP(hipMemcpyAsync(L->ist, hist, sizeof(hist), hipMemcpyHostToDevice, saved));
+    SOD_HIP(hipStreamSynchronize(saved));
+    rt().stream = L->stream;
+    const bool prof = prof_set(false);
+    int rc = SOD_OK;
+    if (!L->exec || !(c == L->cfg)) {
+        loop_free_graph(L);
+        hipError_t e = hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            loop_enqueue_step(L, c);
+            e = hipStreamEndCapture(L->stream, &L->graph);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&L->exec, L->graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) {
+            L->cfg = c;
+        } else {
+            rc = fail(SOD_ERR_HIP, "time_loop_run: graph capture: %s", hipGetErrorString(e));
+            loop_free_graph(L);
+        }
+    }
+    if (rc == SOD_OK) {
+        for (int i = 0; i < nsteps && rc == SOD_OK; ++i) {
+            const hipError_t e = hipGraphLaunch(L->exec, L->stream);
+            if (e != hipSuccess) rc = fail(SOD_ERR_HIP, "time_loop_run: graph launch: %s", hipGetErrorString(e));
+        }
+        const hipError_t e = hipStreamSynchronize(L->stream);
+        if (rc == SOD_OK && e != hipSuccess) rc = fail(SOD_ERR_HIP, "time_loop_run: %s", hipGetErrorString(e));
+    }
+    rt().stream = saved;
+    prof_set(prof);
+    SOD_TRY(rc);
+    double osc[2];
+    SOD_HIP(hipMemcpy(osc, L->sc, sizeof(osc), hipMemcpyDeviceToHost));
+    *dt = osc[0];
+    *time = osc[1];
+    if (nlog > 0) SOD_HIP(hipMemcpy(EK, L->ek_hist, sizeof(double) * nlog, hipMemcpyDeviceToHost));
+    return SOD_OK;
+}
+
+extern "C" int sod_time_loop_destroy(sod_time_loop* L) {
+    if (L == nullptr) return SOD_OK;
+    if (L->stream) (void)hipStreamSynchronize(L->stream);
+    loop_free_graph(L);
+    if (L->stream) (void)hipStreamDestroy(L->stream);
+    (void)hipFree(L->Ngp);
+    (void)hipFree(L->sc);
+    (void)hipFree(L->ist);
+    (void)hipFree(L->ek_partial);
+    (void)hipFree(L->ek_hist);
+    L->P.release();
+    delete L;
+    return SOD_OK;
 }

@@ -37,6 +37,12 @@ int prof_id(const char* name) {
     return (int)g_names.size() - 1;
 }
 
+bool prof_set(bool on) {
+    const bool was = g_on;
+    g_on = on;
+    return was;
+}
+
 ProfScope::ProfScope(int id_) : id(id_), rec(-1) {
     if (!g_on) return;
     hipEvent_t a = get_event(), b = get_event();

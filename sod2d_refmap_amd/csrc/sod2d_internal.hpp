@@ -130,6 +130,7 @@ int scratch(size_t bytes, void** out, int slot);
 // Named timing ranges (HIP events on the library stream), enabled by
 // sod_profile_enable(1).
 int prof_id(const char* name);
+bool prof_set(bool on);   // enables/disables recording, returns the previous state
 struct ProfScope {
     int id, rec;
     explicit ProfScope(int id);

@@ -127,6 +127,56 @@ class PeriodicSolver:
                   ctypes.addressof(ek))
         return ek.value
 
+    # ------------------------------------------------- device time loop
+    def time_loop(self):
+        """The device-resident time loop over this solver's arrays
+        (sod_time_loop_*), created on first use."""
+        import ctypes
+        if getattr(self, "_loop", None) is None:
+            g = self.gas
+            h = ctypes.c_void_p()
+            _lib.call("sod_time_loop_create", ctypes.byref(h), self.nelem, self.npoin, self.npoin_w,
+                      ptr(self.connec), ptr(self.lpoin_w), ptr(self.He), ptr(self.gpvol), ptr(self.Ml),
+                      ptr(self.helem), ptr(self.helem_l), ptr(self.mu_factor), g.Rgas, g.gamma_gas, g.Cp, g.Prt,
+                      ptr(self.lv["rho"]), ptr(self.lv["u"]), ptr(self.lv["q"]), ptr(self.lv["pr"]),
+                      ptr(self.lv["E"]), ptr(self.lv["Tem"]), ptr(self.csound), ptr(self.machno),
+                      ptr(self.lv["e_int"]), ptr(self.lv["eta"]), ptr(self.mu_e), ptr(self.mu_sgs),
+                      ptr(self.mu_fluid), ptr(self.mue_l))
+            self._loop = h
+        return self._loop
+
+    def run_device(self, nsteps: int, istep0: int = 1, log_first: int = 1, log_every: int = 0,
+                   fixed_dt: bool = False, keep_levels: bool = True):
+        """nsteps of the reference's time loop on the device (dt from the CFL
+        condition after each step unless fixed_dt).  Returns the EK values of
+        the log steps."""
+        import ctypes
+        L = self.time_loop()
+        if self.dt == 0.0 and not fixed_dt:
+            self.adapt_dt()
+        nlog = 0
+        if log_every > 0:
+            nlog = sum(1 for i in range(istep0, istep0 + nsteps) if i >= log_first and (i - log_first) % log_every == 0)
+        ek = (ctypes.c_double * max(1, nlog))()
+        dt = ctypes.c_double(self.dt)
+        tm = ctypes.c_double(self.time)
+        _lib.call("sod_time_loop_run", L, istep0, nsteps, self.cfl_conv, self.cfl_diff, int(fixed_dt),
+                  ctypes.byref(dt), ctypes.byref(tm), log_first, log_every, self.gas.rho0, ek, max(1, nlog),
+                  int(keep_levels))
+        self.dt, self.time = dt.value, tm.value
+        return [ek[i] for i in range(nlog)]
+
+    def close(self) -> None:
+        if getattr(self, "_loop", None) is not None:
+            _lib.call("sod_time_loop_destroy", self._loop)
+            self._loop = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
     def run(self, nsteps: int) -> None:
         """evalTimeIteration's step sequence for the periodic case."""
         if self.dt == 0.0:
