@@ -208,6 +208,19 @@ int sod_full_rhs(int nelem, int npoin, int npoin_w, const int *connec, const int
                  const double *mu_sgs, double Rgas, double gamma_gas, double Cp, double Prt,
                  double *Rmass, double *Rmom, double *Rener, double *Reta);
 
+/* Adjoint of sod_full_rhs (no reference counterpart: the reference has no
+ * adjoint).  For seeds lam_* on the four residuals (lam_mom(npoin,3)) it
+ * returns the vector-Jacobian products
+ *   g_rho = lam . dR/drho, g_q(npoin,3) = lam . dR/dq, g_E = lam . dR/dE
+ * at the nodes and g_mu_e(nelem,64) = lam . dR/dmu_e at the element nodes,
+ * with mu_factor and mu_sgs held fixed.  Outputs are overwritten. */
+int sod_full_rhs_vjp(int nelem, int npoin, int npoin_w, const int *connec, const int *lpoin_w,
+                     const double *He, const double *gpvol, const double *Ml, const double *rho,
+                     const double *q, const double *E, const double *mu_factor, const double *mu_e,
+                     const double *mu_sgs, double Rgas, double gamma_gas, double Cp, double Prt,
+                     const double *lam_mass, const double *lam_mom, const double *lam_ener,
+                     const double *lam_eta, double *g_rho, double *g_q, double *g_E, double *g_mu_e);
+
 /* No reference counterpart: selects how sod_rk_4_main evaluates a step.
  * 0 (default): fused stage kernels (one element kernel + one node kernel
  * per RK stage); 1: one kernel per reference routine, in the reference's

@@ -26,7 +26,7 @@
 #include <cstring>
 #include <vector>
 
-#include "sod2d_internal.hpp"
+#include "stage_common.hpp"
 
 namespace sod {
 
@@ -34,50 +34,6 @@ namespace {
 
 constexpr int WPB = 4;   // wavefronts (elements) per workgroup
 constexpr int BLK = 256;
-
-__device__ __forceinline__ int swz(int i, int j, int k) {
-    return 16 * k + ((i + k) & 3) + 4 * ((j + k) & 3);
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Per-stage constants; ratios of the gas constants are formed once on the
-// host so that the element kernel divides only by node values.
-struct StageParams {
-    double gamma, gm1, ig1, gog1, iRgas, CpPr, Cp09, c_rho, c_ener;
-};
-
-StageParams make_stage_params(double gamma, double Rgas, double Cp, double Pr, double c_rho, double c_ener) {
-    StageParams P;
-    P.gamma = gamma;
-    P.gm1 = gamma - 1.0;
-    P.ig1 = 1.0 / (gamma - 1.0);
-    P.gog1 = gamma / (gamma - 1.0);
-    P.iRgas = 1.0 / Rgas;
-    P.CpPr = Cp / Pr;
-    P.Cp09 = Cp / 0.9;   // turbulent Prandtl number 0.9 (elem_diffu.f90)
-    P.c_rho = c_rho;
-    P.c_ener = c_ener;
-    return P;
-}
-
-enum { P_RHO, P_U0, P_U1, P_U2, P_P, P_H, P_K, P_ETA, P_T, NPRIM };
-enum { R_MASS, R_MOM0, R_MOM1, R_MOM2, R_ENER, R_ETA, NRES };
 
 // ------------------------------------------------------------ element kernel
 // Along each direction the four points of a lane's line are visited as the
@@ -856,6 +812,17 @@ int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const i
     hipLaunchKernelGGL(k_set_double_f, dim3(1), dim3(64), 0, rt().stream, P.dscal, dt);
     enqueue_fused_step(P, a, P.dscal, has_sgs != 0, false);
     SOD_HIP(hipGetLastError());
+    return SOD_OK;
+}
+
+int prepare_lane_mesh(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
+                      const double* He, const double* gpvol, const double* Ml, const double* mu_e,
+                      const double* mu_sgs, LaneMesh* out) {
+    Prepared& P = g_prep;
+    SOD_TRY(prepare(P, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
+    int has_sgs = 0;
+    SOD_TRY(load_viscosities(P, nelem, mu_e, mu_sgs, &has_sgs));
+    *out = LaneMesh{P.conn, P.He, P.gpv, P.mue, P.musgs, has_sgs != 0};
     return SOD_OK;
 }
 

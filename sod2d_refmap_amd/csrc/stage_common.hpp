@@ -1,0 +1,70 @@
+// Pieces shared by the fused stage kernels (k_fused.hip) and their adjoint
+// (k_adjoint.hip): lattice-lane helpers, stage constants, field indices and
+// the lane-layout view of a prepared mesh.
+#pragma once
+
+#include "sod2d_internal.hpp"
+
+namespace sod {
+
+// LDS slot of lattice point (i,j,k): conflict-free for the 4-point line
+// reads along each of the three directions
+__device__ __forceinline__ int swz(int i, int j, int k) {
+    return 16 * k + ((i + k) & 3) + 4 * ((j + k) & 3);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// orders LDS accesses of the lanes of one wavefront
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-stage constants; ratios of the gas constants are formed once on the
+// host so that the element kernels divide only by node values.
+struct StageParams {
+    double gamma, gm1, ig1, gog1, iRgas, CpPr, Cp09, c_rho, c_ener;
+};
+
+inline StageParams make_stage_params(double gamma, double Rgas, double Cp, double Pr, double c_rho,
+                                     double c_ener) {
+    StageParams P;
+    P.gamma = gamma;
+    P.gm1 = gamma - 1.0;
+    P.ig1 = 1.0 / (gamma - 1.0);
+    P.gog1 = gamma / (gamma - 1.0);
+    P.iRgas = 1.0 / Rgas;
+    P.CpPr = Cp / Pr;
+    P.Cp09 = Cp / 0.9;   // turbulent Prandtl number 0.9 (elem_diffu.f90)
+    P.c_rho = c_rho;
+    P.c_ener = c_ener;
+    return P;
+}
+
+enum { P_RHO, P_U0, P_U1, P_U2, P_P, P_H, P_K, P_ETA, P_T, NPRIM };
+enum { R_MASS, R_MOM0, R_MOM1, R_MOM2, R_ENER, R_ETA, NRES };
+
+// Lane-layout view of the mesh prepared for the drop-in entry points
+// (k_fused.hip): conn[e][lane] (0-based), He[e][d+3c][lane], gpv/mue/musgs
+// [e][lane]; mue/musgs hold the caller's mu_e / mu_sgs of the last call.
+struct LaneMesh {
+    const int* conn;
+    const double *He, *gpv, *mue, *musgs;
+    bool has_sgs;
+};
+int prepare_lane_mesh(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
+                      const double* He, const double* gpvol, const double* Ml, const double* mu_e,
+                      const double* mu_sgs, LaneMesh* out);
+
+}  // namespace sod

@@ -1,0 +1,117 @@
+"""The hand-written adjoint of the assembled stage RHS (sod_full_rhs_vjp)
+against autograd through the FP64 torch transcription of the reference
+element routines (tests/torch_rhs.py, itself checked against the oracle),
+and against central finite differences of the forward library call."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import torch_rhs as TR
+from _util import mesh_geometry, rel_l2
+from sod2d_refmap_amd import _lib, cases, mesh as pmesh
+from sod2d_refmap_amd.device import to_device, to_host
+
+pytestmark = pytest.mark.gpu
+
+T = oracle.hex64_tables()
+
+
+def _setup(n, sgs, seed):
+    m = pmesh.periodic_box(n, curved_amp=0.05, perm_seed=seed)
+    g = mesh_geometry(m)
+    gas = cases.GasParams()
+    s = cases.tgv_initial_state(m, gas, perturb_seed=seed + 1, perturb_amp=5e-2)
+    rng = np.random.default_rng(seed + 2)
+    mu_e = np.asfortranarray(rng.uniform(0, 2e-3, (m.nelem, 64)))
+    mu_sgs = np.asfortranarray(rng.uniform(0, 1e-3, (m.nelem, 64)) if sgs else np.zeros((m.nelem, 64)))
+    lam = [rng.standard_normal(m.npoin), np.asfortranarray(rng.standard_normal((m.npoin, 3))),
+           rng.standard_normal(m.npoin), rng.standard_normal(m.npoin)]
+    return m, g, gas, s, mu_e, mu_sgs, lam
+
+
+class Lib:
+    def __init__(self, m, g, gas, s, mu_e, mu_sgs):
+        self.m, self.gas = m, gas
+        self.keep = {}
+        d = lambda k, a: self.keep.setdefault(k, to_device(a)).data_ptr()
+        self.static = [d("c", m.connec), d("lw", m.lpoin_w), d("He", g["He"]), d("gv", g["gpvol"]),
+                       d("Ml", g["Ml"])]
+        self.mf, self.mue, self.sgs = d("mf", s["mu_factor"]), d("mue", mu_e), d("sgs", mu_sgs)
+
+    def _o(self, n):
+        return torch.zeros(n, dtype=torch.float64, device="cuda")
+
+    def rhs(self, rho, q, E):
+        m, gas = self.m, self.gas
+        r, qq, e = to_device(rho), to_device(q), to_device(E)
+        out = [self._o(m.npoin), self._o(3 * m.npoin), self._o(m.npoin), self._o(m.npoin)]
+        _lib.call("sod_full_rhs", m.nelem, m.npoin, m.npoin_w, *self.static, r.data_ptr(), qq.data_ptr(),
+                  e.data_ptr(), self.mf, self.mue, self.sgs, gas.Rgas, gas.gamma_gas, gas.Cp, gas.Prt,
+                  *[o.data_ptr() for o in out])
+        return [to_host(out[0], (m.npoin,)), to_host(out[1], (m.npoin, 3)), to_host(out[2], (m.npoin,)),
+                to_host(out[3], (m.npoin,))]
+
+    def vjp(self, rho, q, E, lam):
+        m, gas = self.m, self.gas
+        r, qq, e = to_device(rho), to_device(q), to_device(E)
+        ls = [to_device(x) for x in lam]
+        out = [self._o(m.npoin), self._o(3 * m.npoin), self._o(m.npoin), self._o(64 * m.nelem)]
+        _lib.call("sod_full_rhs_vjp", m.nelem, m.npoin, m.npoin_w, *self.static, r.data_ptr(), qq.data_ptr(),
+                  e.data_ptr(), self.mf, self.mue, self.sgs, gas.Rgas, gas.gamma_gas, gas.Cp, gas.Prt,
+                  *[x.data_ptr() for x in ls], *[o.data_ptr() for o in out])
+        return [to_host(out[0], (m.npoin,)), to_host(out[1], (m.npoin, 3)), to_host(out[2], (m.npoin,)),
+                to_host(out[3], (m.nelem, 64))]
+
+
+def _autograd(m, g, gas, s, mu_e, mu_sgs, lam):
+    geo = TR.Geometry(m.connec, g["He"], g["gpvol"], T)
+    t = lambda a: torch.tensor(np.asarray(a), dtype=torch.float64, requires_grad=True)
+    tr, tq, tE = t(s["rho"]), t(s["q"]), t(s["E"])
+    tmue = TR.lattice_from_elemnode(geo, mu_e).requires_grad_(True)
+    rm, rq, re, rt = TR.full_rhs(geo, tr, tq, tE, torch.tensor(s["mu_factor"]), tmue,
+                                 TR.lattice_from_elemnode(geo, mu_sgs), gas.gamma_gas, gas.Rgas, gas.Cp, gas.Prt)
+    L = ((rm * torch.tensor(lam[0])).sum() + (rq * torch.tensor(lam[1])).sum() +
+         (re * torch.tensor(lam[2])).sum() + (rt * torch.tensor(lam[3])).sum())
+    L.backward()
+    # lattice [i,j,k] (i-major) -> gmsh node order
+    lat = geo.lat.numpy()
+    gm = np.zeros((m.nelem, 64))
+    gm[:, lat] = tmue.grad.reshape(m.nelem, 64).numpy()
+    return [tr.grad.numpy(), tq.grad.numpy(), tE.grad.numpy(), gm]
+
+
+@pytest.mark.parametrize("n,sgs", [(2, False), (3, True)], ids=["curved2", "curved3_sgs"])
+def test_vjp_matches_autograd(gpu, n, sgs):
+    m, g, gas, s, mu_e, mu_sgs, lam = _setup(n, sgs, 7 * n)
+    print(f"[vjp] n={n} sgs={sgs} npoin={m.npoin}", flush=True)
+    L = Lib(m, g, gas, s, mu_e, mu_sgs)
+    got = L.vjp(s["rho"], s["q"], s["E"], lam)
+    ref = _autograd(m, g, gas, s, mu_e, mu_sgs, lam)
+    for name, a, b in zip(("rho", "q", "E", "mu_e"), got, ref):
+        err = rel_l2(a, b)
+        print(f"  vjp {name} rel_l2={err:.3e}", flush=True)
+        assert err < 1e-11, (name, err)
+
+
+def test_vjp_central_differences(gpu):
+    """lam . R(U + h v) - lam . R(U - h v) over 2h against g . v along seeded
+    random directions v in (rho, q, E)."""
+    m, g, gas, s, mu_e, mu_sgs, lam = _setup(2, False, 5)
+    L = Lib(m, g, gas, s, mu_e, mu_sgs)
+    gr, gq, gE, _ = L.vjp(s["rho"], s["q"], s["E"], lam)
+    rng = np.random.default_rng(42)
+    dot = lambda R: sum(float(np.sum(a * b)) for a, b in zip(R, lam))
+    for it in range(4):
+        v = [rng.standard_normal(m.npoin), rng.standard_normal((m.npoin, 3)), rng.standard_normal(m.npoin)]
+        scale = [np.abs(s["rho"]).max(), np.abs(s["q"]).max(), np.abs(s["E"]).max()]
+        v = [a * sc for a, sc in zip(v, scale)]
+        nv = np.sqrt(sum(float(np.sum(a * a)) for a in v))
+        v = [a / nv for a in v]
+        h = 1e-6 * np.sqrt(float(np.sum(s["E"] ** 2)))
+        up = L.rhs(s["rho"] + h * v[0], np.asfortranarray(s["q"] + h * v[1]), s["E"] + h * v[2])
+        dn = L.rhs(s["rho"] - h * v[0], np.asfortranarray(s["q"] - h * v[1]), s["E"] - h * v[2])
+        fd = (dot(up) - dot(dn)) / (2 * h)
+        an = float(np.sum(gr * v[0]) + np.sum(gq * v[1]) + np.sum(gE * v[2]))
+        print(f"  fd direction {it}: fd={fd:.10e} adjoint={an:.10e}", flush=True)
+        assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
