@@ -221,6 +221,21 @@ int sod_full_rhs_vjp(int nelem, int npoin, int npoin_w, const int *connec, const
                      const double *lam_mass, const double *lam_mom, const double *lam_ener,
                      const double *lam_eta, double *g_rho, double *g_q, double *g_E, double *g_mu_e);
 
+/* Adjoint of sod_smart_visc_spectral (no reference counterpart).  For a
+ * seed lam_mu_e(nelem,64) on mu_e it returns lam . dmu_e/dx for
+ * x = rho, u(npoin,3), csound, Tem, eta, Reta (nodal; eta enters through the
+ * normalisation max |eta - mean(eta)| over lpoin_w).  The limiters are
+ * differentiated as subgradients: min(Ve, beta_e) follows Ve where
+ * Ve < beta_e; each max over nodes follows its first maximiser (lowest gmsh
+ * node, or first lpoin_w entry); max(x, ce) passes the gradient only when
+ * x > ce.  Outputs are overwritten. */
+int sod_smart_visc_spectral_vjp(int nelem, int npoin, int npoin_w, const int *connec, const int *lpoin_w,
+                                const double *Reta, const double *gpvol, double gamma_gas, const double *rho,
+                                const double *u, const double *csound, const double *Tem, const double *eta,
+                                const double *helem, const double *helem_k, const double *lam_mu_e,
+                                double *g_rho, double *g_u, double *g_csound, double *g_Tem, double *g_eta,
+                                double *g_Reta);
+
 /* No reference counterpart: selects how sod_rk_4_main evaluates a step.
  * 0 (default): fused stage kernels (one element kernel + one node kernel
  * per RK stage); 1: one kernel per reference routine, in the reference's

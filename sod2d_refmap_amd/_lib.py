@@ -69,6 +69,7 @@ _SIGS = {
     "sod_set_rk4_path": [c_int],
     "sod_full_rhs": [c_int] * 3 + [P] * 11 + [c_double] * 4 + [P] * 4,
     "sod_full_rhs_vjp": [c_int] * 3 + [P] * 11 + [c_double] * 4 + [P] * 8,
+    "sod_smart_visc_spectral_vjp": [c_int] * 3 + [P] * 4 + [c_double] + [P] * 14,
     "sod_rk_4_main": ([c_int] * 7 + [P] * 9 + [c_int] + [P] * 8 + [c_double] + [P] * 2 +
                       [c_double] * 4 + [P] * 22),
     "sod_time_loop_create": [ctypes.POINTER(c_void_p)] + [c_int] * 3 + [P] * 9 + [c_double] * 4 + [P] * 14,

@@ -28,6 +28,12 @@ namespace sod {
 namespace {
 
 constexpr int AWPB = 2;   // wavefronts (elements) per workgroup
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
 constexpr int NROW = 50;  // LDS rows per element
 
 // published rows of the transposed pass
@@ -471,10 +477,241 @@ __global__ __launch_bounds__(256) void k_zero_doubles(long n, double* __restrict
     for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) p[t] = 0.0;
 }
 
+// ---------------------------------------------------- entropy viscosity
+// Adjoint of smart_visc_spectral (mod_entropy_viscosity.f90:68-122 and the
+// filter of mod_filters.f90:15-55): for a seed lambda on mu_e(nelem,64) it
+// accumulates lambda . dmu_e/d(rho, u, csound, Tem, Reta) at the nodes and
+// lambda . dmu_e/d(norm) into g_norm.  The limiters are differentiated as
+// subgradients: min(Ve, beta_e) follows Ve where Ve < beta_e, else beta_e;
+// every max over the element's nodes follows its first (lowest-lane)
+// maximiser; the max(.., ce) clamps pass the gradient only when the clamped
+// argument is strictly larger.  The element's Jacobian extent (minJe/maxJe)
+// is geometry and carries no gradient.
+__device__ __forceinline__ int first_lane(unsigned long long m) { return __ffsll((long long)m) - 1; }
+
+__global__ __launch_bounds__(256) void k_smart_visc_vjp(
+    int nelem, int npoin, const int* __restrict__ connec, const double* __restrict__ Reta,
+    const double* __restrict__ gpvol, double gamma_gas, const double* __restrict__ rho,
+    const double* __restrict__ u, const double* __restrict__ csound, const double* __restrict__ Tem,
+    const double* __restrict__ helem_l, const double* __restrict__ helem, const double* __restrict__ d_stats,
+    int normalise, double Rgas, double ce, double cmax, double cglob, const double* __restrict__ lam_mu_e,
+    double* __restrict__ g, double* __restrict__ g_norm) {
+    __shared__ double s_lam[4][64];   // seed of mue_l, by gmsh node
+    const int le = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int e_raw = blockIdx.x * 4 + le;
+    const bool active = e_raw < nelem;
+    const int e = active ? e_raw : nelem - 1;   // inactive waves mirror a valid element, write nothing
+    const size_t np = (size_t)npoin, ne = (size_t)nelem;
+    const double norm = normalise ? d_stats[1] : 1.0;
+    const int a = l;
+    const int n = connec[(size_t)e + ne * a] - 1;
+    const double J = gpvol[a + 64 * (size_t)e] / c_tab.wgp[a];
+    const double uu[3] = {u[n], u[n + np], u[n + 2 * np]};
+    const double vmag = sqrt(uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2]);
+    const double cs = csound[n], rr = rho[n], TT = Tem[n], Rt = Reta[n];
+    const double minJe = wave_min(fmin(1000000.0, J));
+    const double maxJe = wave_max(fmax(0.0, J));
+    const double maxV = wave_max(vmag);
+    const double maxC = wave_max(cs);
+    const double M = maxV / maxC;
+    double M15 = M * M;
+    M15 = M15 * M15 * M;
+    M15 = M15 * M15 * M15;
+    const double v_pi = 3.14159265358979323846;
+    const double th = tanh(M15 * v_pi);
+    const double ceM = fmax(th, ce);
+    const double rj = minJe / maxJe;
+    const double ced_geo = fmax(1.0 - rj * rj, ce);
+    const double ced = fmax(ced_geo, ceM);
+    const double aux3 = sqrt(Rgas * gamma_gas * TT);
+    const double he = helem[e];
+    const double beta_l = (rr * he) * (cmax / 3.0) * (vmag + aux3);
+    const double betae = wave_max(beta_l);
+    const double R1 = rr * fabs(Rt) / norm;
+    const double hl = helem_l[(size_t)e + ne * a];
+    const double Ve = ced * R1 * (hl * hl);
+
+    // transpose of the 27-point mirror filter (same position/weight table as
+    // the forward): lane = geometric position, scatter into its inputs
+    s_lam[le][l] = 0.0;
+    __syncthreads();
+    {
+        const int conv[6] = {3, 1, 3, 4, 2, 3};
+        const int gi = (l & 3) + 1, gj = ((l >> 2) & 3) + 1, gk = (l >> 4) + 1;
+        const double w3[3] = {0.25, 0.5, 0.25};
+        const int so = c_tab.lane2node[(conv[gi] - 1) + 4 * (conv[gj] - 1) + 16 * (conv[gk] - 1)];
+        const double lm = active ? lam_mu_e[(size_t)e + ne * so] : 0.0;
+#pragma unroll
+        for (int ll = -1; ll <= 1; ++ll)
+#pragma unroll
+            for (int mm = -1; mm <= 1; ++mm)
+#pragma unroll
+                for (int nn = -1; nn <= 1; ++nn) {
+                    const int si = conv[gi + ll] - 1, sj = conv[gj + mm] - 1, sk = conv[gk + nn] - 1;
+                    const int b = c_tab.lane2node[si + 4 * sj + 16 * sk];
+                    atomicAdd(&s_lam[le][b], w3[ll + 1] * w3[mm + 1] * w3[nn + 1] * lm);
+                }
+    }
+    __syncthreads();
+    if (!active) return;
+    const double lml = cglob * s_lam[le][a];
+    const bool via_ve = Ve < betae;
+    const double L_Ve = via_ve ? lml : 0.0;
+    const double L_betae = wave_sum_d(via_ve ? 0.0 : lml);
+    const double L_ced = wave_sum_d(L_Ve * R1 * (hl * hl));
+    const double L_R1 = L_Ve * ced * (hl * hl);
+    double Lr = L_R1 * fabs(Rt) / norm;
+    const double sgn = Rt > 0.0 ? 1.0 : (Rt < 0.0 ? -1.0 : 0.0);
+    double LRt = L_R1 * rr * sgn / norm;
+    const double Lnorm = normalise ? wave_sum_d(-L_R1 * R1 / norm) : 0.0;
+    double Lvmag = 0.0, Lcs = 0.0, LT = 0.0;
+    // ced = max(ced_geo, ceM), ceM = max(tanh(pi M^15), ce), M = maxV / maxC
+    if (ceM > ced_geo && th > ce && M > 0.0) {
+        double M7 = M * M * M;
+        M7 = M7 * M7 * M;
+        const double L_M = L_ced * v_pi * (1.0 - th * th) * 15.0 * (M7 * M7);
+        const int iv = first_lane(__ballot(vmag == maxV));
+        const int ic = first_lane(__ballot(cs == maxC));
+        if (l == iv) Lvmag += L_M / maxC;
+        if (l == ic) Lcs += -L_M * maxV / (maxC * maxC);
+    }
+    // beta_e = max over nodes of rho h_e cmax/3 (|u| + sqrt(R gamma T))
+    {
+        const int ib = first_lane(__ballot(beta_l == betae));
+        if (l == ib) {
+            const double k3 = he * (cmax / 3.0);
+            Lr += L_betae * k3 * (vmag + aux3);
+            Lvmag += L_betae * rr * k3;
+            const double Laux3 = L_betae * rr * k3;
+            if (aux3 > 0.0) LT += Laux3 * Rgas * gamma_gas / (2.0 * aux3);
+        }
+    }
+    double Lu[3] = {0.0, 0.0, 0.0};
+    if (vmag > 0.0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) Lu[d] = Lvmag * uu[d] / vmag;
+    }
+    unsafeAtomicAdd(g + n, Lr);
+    unsafeAtomicAdd(g + np + n, Lu[0]);
+    unsafeAtomicAdd(g + 2 * np + n, Lu[1]);
+    unsafeAtomicAdd(g + 3 * np + n, Lu[2]);
+    unsafeAtomicAdd(g + 4 * np + n, Lcs);
+    unsafeAtomicAdd(g + 5 * np + n, LT);
+    unsafeAtomicAdd(g + 6 * np + n, LRt);
+    if (l == 0 && normalise) unsafeAtomicAdd(g_norm, Lnorm);
+}
+
+// norm = max over working nodes of |eta - mean|: the first working node
+// (in lpoin_w order) attaining it
+__global__ __launch_bounds__(256) void k_norm_argmax(int npoin_w, const int* __restrict__ lpoin_w,
+                                                     const double* __restrict__ eta,
+                                                     const double* __restrict__ d_stats, int* __restrict__ idx) {
+    const double mean = d_stats[0] / (double)npoin_w;
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < npoin_w; t += gridDim.x * 256)
+        if (fabs(eta[lpoin_w[t] - 1] - mean) == d_stats[1]) atomicMin(idx, t);
+}
+
+// lambda_eta = g_norm * s * (delta(n, n*) - 1/npoin_w) over working nodes
+__global__ __launch_bounds__(256) void k_norm_to_eta(int npoin_w, const int* __restrict__ lpoin_w,
+                                                     const double* __restrict__ eta,
+                                                     const double* __restrict__ d_stats,
+                                                     const int* __restrict__ idx,
+                                                     const double* __restrict__ g_norm, double* __restrict__ g_eta) {
+    const int istar = *idx;
+    if (istar < 0 || istar >= npoin_w) return;
+    const double mean = d_stats[0] / (double)npoin_w;
+    const double dv = eta[lpoin_w[istar] - 1] - mean;
+    const double sgn = dv > 0.0 ? 1.0 : (dv < 0.0 ? -1.0 : 0.0);
+    const double Ln = g_norm[0] * sgn;
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < npoin_w; t += gridDim.x * 256) {
+        const int n = lpoin_w[t] - 1;
+        g_eta[n] = (t == istar ? Ln : 0.0) - Ln / (double)npoin_w;
+    }
+}
+
+__global__ void k_init_vjp_scalars(double* g_norm, int* idx, int big) {
+    if (threadIdx.x == 0) {
+        *g_norm = 0.0;
+        *idx = big;
+    }
+}
+
 }  // namespace
+
+void launch_eta_stats(int npoin_w, const int* lpoin_w, const double* eta, double* work);
+
 }  // namespace sod
 
 using namespace sod;
+
+extern "C" int sod_smart_visc_spectral_vjp(int nelem, int npoin, int npoin_w, const int* connec,
+                                           const int* lpoin_w, const double* Reta, const double* gpvol,
+                                           double gamma_gas, const double* rho, const double* u,
+                                           const double* csound, const double* Tem, const double* eta,
+                                           const double* helem_l, const double* helem, const double* lam_mu_e,
+                                           double* g_rho, double* g_u, double* g_csound, double* g_Tem,
+                                           double* g_eta, double* g_Reta) {
+    SOD_TRY(ensure_init());
+    if (nelem < 0 || npoin < 0 || npoin_w < 0 || npoin_w > npoin)
+        return fail(SOD_ERR_ARG, "smart_visc_spectral_vjp: inconsistent sizes");
+    const int *dc, *dl;
+    const double *dRt, *dgv, *dr, *du, *dcs, *dT, *deta, *dhl, *dh, *dlam;
+    double *ogr, *ogu, *ogc, *ogT, *oge, *ogR;
+    SOD_TRY(resolve_t(connec, "connec", &dc));
+    SOD_TRY(resolve_t(lpoin_w, "lpoin_w", &dl));
+    SOD_TRY(resolve_t(Reta, "Reta", &dRt));
+    SOD_TRY(resolve_t(gpvol, "gpvol", &dgv));
+    SOD_TRY(resolve_t(rho, "rho", &dr));
+    SOD_TRY(resolve_t(u, "u", &du));
+    SOD_TRY(resolve_t(csound, "csound", &dcs));
+    SOD_TRY(resolve_t(Tem, "Tem", &dT));
+    SOD_TRY(resolve_t(eta, "eta", &deta));
+    SOD_TRY(resolve_t(helem_l, "helem_l", &dhl));
+    SOD_TRY(resolve_t(helem, "helem", &dh));
+    SOD_TRY(resolve_t(lam_mu_e, "lam_mu_e", &dlam));
+    SOD_TRY(resolve_w(g_rho, "g_rho", &ogr));
+    SOD_TRY(resolve_w(g_u, "g_u", &ogu));
+    SOD_TRY(resolve_w(g_csound, "g_csound", &ogc));
+    SOD_TRY(resolve_w(g_Tem, "g_Tem", &ogT));
+    SOD_TRY(resolve_w(g_eta, "g_eta", &oge));
+    SOD_TRY(resolve_w(g_Reta, "g_Reta", &ogR));
+    const sod_numerical_params& prm = rt().params;
+    hipStream_t s = rt().stream;
+    const size_t np = (size_t)npoin;
+    constexpr int RB = 1024;
+    void* buf = nullptr;
+    SOD_TRY(scratch(sizeof(double) * (np * 7 + 2 * RB + 16), &buf, 7));
+    double* g = static_cast<double*>(buf);
+    double* stats = g + 7 * np;          // [0] sum, [1] norm, partials
+    double* g_norm = stats + 2 * RB + 4;
+    int* idx = reinterpret_cast<int*>(g_norm + 2);
+    const int normalise = prm.flag_normalise_entropy == 1;
+    hipLaunchKernelGGL(k_zero_doubles, dim3(std::max(1, std::min(1024, grid_for((long)np * 7, 256)))), dim3(256), 0, s,
+                       (long)np * 7, g);
+    hipLaunchKernelGGL(k_init_vjp_scalars, dim3(1), dim3(64), 0, s, g_norm, idx, npoin_w);
+    if (normalise && npoin_w > 0) launch_eta_stats(npoin_w, dl, deta, stats);
+    if (nelem > 0) {
+        SOD_PROF("smart_visc_vjp");
+        hipLaunchKernelGGL(k_smart_visc_vjp, dim3(grid_for(nelem, 4)), dim3(256), 0, s, nelem, npoin, dc, dRt, dgv,
+                           gamma_gas, dr, du, dcs, dT, dhl, dh, stats, normalise, prm.Rgas_inf, prm.ce, prm.cmax,
+                           prm.cglob, dlam, g, g_norm);
+    }
+    // eta enters only through the normalisation
+    hipLaunchKernelGGL(k_zero_doubles, dim3(std::max(1, std::min(1024, grid_for((long)np, 256)))), dim3(256), 0, s,
+                       (long)np, oge);
+    if (normalise && npoin_w > 0) {
+        const int gr = std::max(1, std::min(1024, grid_for(npoin_w, 256)));
+        hipLaunchKernelGGL(k_norm_argmax, dim3(gr), dim3(256), 0, s, npoin_w, dl, deta, stats, idx);
+        hipLaunchKernelGGL(k_norm_to_eta, dim3(gr), dim3(256), 0, s, npoin_w, dl, deta, stats, idx, g_norm, oge);
+    }
+    SOD_HIP(hipMemcpyAsync(ogr, g, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+    SOD_HIP(hipMemcpyAsync(ogu, g + np, sizeof(double) * np * 3, hipMemcpyDeviceToDevice, s));
+    SOD_HIP(hipMemcpyAsync(ogc, g + 4 * np, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+    SOD_HIP(hipMemcpyAsync(ogT, g + 5 * np, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+    SOD_HIP(hipMemcpyAsync(ogR, g + 6 * np, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+    SOD_HIP(hipGetLastError());
+    return finish_call();
+}
 
 extern "C" int sod_full_rhs_vjp(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
                                 const double* He, const double* gpvol, const double* Ml, const double* rho,

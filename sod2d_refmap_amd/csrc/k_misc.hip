@@ -409,6 +409,18 @@ void launch_smart_visc(int nelem, int npoin, int npoin_w, const int* connec, con
                            mue_l);
 }
 
+// Sum and maximum deviation of eta over the working nodes
+// (mod_entropy_viscosity.f90:43-66): work[0] = sum, work[1] = max |eta -
+// sum/npoin_w|; `work` holds >= 2 * RED_BLOCKS + 2 doubles.
+void launch_eta_stats(int npoin_w, const int* lpoin_w, const double* eta, double* work) {
+    hipStream_t s = rt().stream;
+    double* partial = work + 2;
+    hipLaunchKernelGGL(k_eta_partial_sum, dim3(RED_BLOCKS), dim3(BLK), 0, s, npoin_w, lpoin_w, eta, partial);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(BLK), 0, s, RED_BLOCKS, partial, work);
+    hipLaunchKernelGGL(k_eta_partial_dev, dim3(RED_BLOCKS), dim3(BLK), 0, s, npoin_w, lpoin_w, eta, work, partial);
+    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(BLK), 0, s, RED_BLOCKS, partial, work + 1);
+}
+
 // Entropy normalisation after a fused step (mod_entropy_viscosity.f90:43-66):
 // work[1] = max |eta - mean(eta)| over working nodes, from `npartial` block
 // partial sums of eta.  `work` holds >= RED_BLOCKS + 2 doubles.

@@ -115,3 +115,124 @@ def test_vjp_central_differences(gpu):
         an = float(np.sum(gr * v[0]) + np.sum(gq * v[1]) + np.sum(gE * v[2]))
         print(f"  fd direction {it}: fd={fd:.10e} adjoint={an:.10e}", flush=True)
         assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+def test_smart_visc_vjp_matches_autograd(gpu):
+    """sod_smart_visc_spectral_vjp against autograd through the torch
+    transcription of smart_visc_spectral (first-index subgradients)."""
+    m, g, gas, s, mu_e, mu_sgs, lam = _setup(3, False, 11)
+    from sod2d_refmap_amd.device import numerical_params, set_numerical_params
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    rng = np.random.default_rng(17)
+    Reta = rng.uniform(-1e-2, 1e-2, m.npoin)
+    lam_mu = np.asfortranarray(rng.standard_normal((m.nelem, 64)))
+    t = lambda a: torch.tensor(np.asarray(a), dtype=torch.float64, requires_grad=True)
+    tr, tu, tc, tT, te, tR = t(s["rho"]), t(s["u"]), t(s["csound"]), t(s["Tem"]), t(s["eta"]), t(Reta)
+    mu, _ = TR.smart_visc(m.connec, m.lpoin_w, tR, g["gpvol"], T.wgp, gas.gamma_gas, tr, tu, tc, tT, te,
+                          g["helem_l"], g["helem"], T.invAtoIJK, gas.Rgas)
+    (mu * torch.tensor(lam_mu)).sum().backward()
+    ref = [tr.grad.numpy(), tu.grad.numpy(), tc.grad.numpy(), tT.grad.numpy(), te.grad.numpy(), tR.grad.numpy()]
+    keep = []
+
+    def d(a):
+        x = to_device(a)
+        keep.append(x)
+        return x.data_ptr()
+
+    outs = [torch.zeros(k * m.npoin, dtype=torch.float64, device="cuda") for k in (1, 3, 1, 1, 1, 1)]
+    _lib.call("sod_smart_visc_spectral_vjp", m.nelem, m.npoin, m.npoin_w, d(m.connec), d(m.lpoin_w), d(Reta),
+              d(g["gpvol"]), gas.gamma_gas, d(s["rho"]), d(s["u"]), d(s["csound"]), d(s["Tem"]), d(s["eta"]),
+              d(g["helem_l"]), d(g["helem"]), d(lam_mu), *[o.data_ptr() for o in outs])
+    shapes = [(m.npoin,), (m.npoin, 3), (m.npoin,), (m.npoin,), (m.npoin,), (m.npoin,)]
+    for name, o, shp, r in zip(("rho", "u", "csound", "Tem", "eta", "Reta"), outs, shapes, ref):
+        got = to_host(o, shp)
+        nr = np.linalg.norm(r)
+        err = np.linalg.norm(got - r) / (nr if nr > 0 else 1.0)
+        print(f"  smart_visc vjp {name} |ref|={nr:.3e} rel_err={err:.3e}", flush=True)
+        assert err < 1e-11, (name, err)
+
+
+def _ev_context(m, g, gas, s):
+    from sod2d_refmap_amd.autograd import EVContext, RHSContext
+    rc = RHSContext.from_mesh(m, g["He"], g["gpvol"], g["Ml"], s["mu_factor"], gas)
+    return EVContext.create(rc, m, g["helem"], g["helem_l"], T)
+
+
+def _cuda(a):
+    return torch.tensor(np.asarray(a), dtype=torch.float64, device="cuda")
+
+
+def test_rk4_chain_matches_rk_4_main(gpu):
+    """The differentiable step (FullRHS stages + final EOS + SmartVisc) is
+    the same map as sod_rk_4_main."""
+    from sod2d_refmap_amd.autograd import rk4_step_ev
+    from sod2d_refmap_amd.device import numerical_params, set_numerical_params
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    m = pmesh.periodic_box(3, curved_amp=0.05, perm_seed=2)
+    g = mesh_geometry(m)
+    gas = cases.GasParams()
+    s = cases.tgv_initial_state(m, gas, perturb_seed=4, perturb_amp=5e-2)
+    s["mu_e"] = np.asfortranarray(np.random.default_rng(3).uniform(0, 1e-3, (m.nelem, 64)))
+    sol = PeriodicSolver(m, gas, s)
+    dt = sol.adapt_dt()
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    ec = _ev_context(m, g, gas, s)
+    y = (_cuda(s["rho"]), _cuda(s["q"]), _cuda(s["E"]), _cuda(s["mu_e"]))
+    for _ in range(2):
+        sol.rk_4_main(dt)
+        y = rk4_step_ev(ec, *y, dt)
+    st = sol.state()
+    w = m.lpoin_w - 1
+    for name, a, b in (("rho", y[0], st["rho"]), ("q", y[1], st["q"]), ("E", y[2], st["E"]),
+                       ("mu_e", y[3], st["mu_e"])):
+        a = a.detach().cpu().numpy()
+        err = rel_l2(a[w] if name != "mu_e" else a, b[w] if name != "mu_e" else b)
+        print(f"  chain vs rk_4_main {name} rel_l2={err:.3e}", flush=True)
+        assert err < 1e-12, (name, err)
+
+
+def test_rk4_adjoint_checkpointed_fd(gpu):
+    """Gradient of the kinetic energy after K RK4 steps with respect to the
+    initial velocity (at fixed density and pressure), through the hand-written
+    adjoints with checkpointing, against central differences of the forward
+    chain (same dt sequence) along seeded random unit directions."""
+    from sod2d_refmap_amd.autograd import kinetic_energy, rk4_step_ev, run_checkpointed
+    from sod2d_refmap_amd.device import numerical_params, set_numerical_params
+    m = pmesh.periodic_box(2, curved_amp=0.05, perm_seed=6)
+    g = mesh_geometry(m)
+    gas = cases.GasParams()
+    s = cases.tgv_initial_state(m, gas, perturb_seed=8, perturb_amp=5e-2)
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    ec = _ev_context(m, g, gas, s)
+    rho0, p0, u0 = _cuda(s["rho"]), _cuda(s["pr"]), _cuda(s["u"])
+    g1 = gas.gamma_gas - 1.0
+
+    def initial(u):
+        return rho0 * 0 + rho0, rho0[:, None] * u, p0 / g1 + 0.5 * rho0 * (u * u).sum(-1)
+
+    dt = oracle.adapt_dt_cfl(m.connec, g["helem"], s["u"], s["csound"], 0.5, 0.5, s["mu_fluid"],
+                             np.zeros((m.nelem, 64), order="F"), s["rho"])
+    K, every = 6, 2
+    dts = [dt] * K
+    mu0 = _cuda(np.zeros((m.nelem, 64)))
+    u = u0.clone().requires_grad_(True)
+    J = run_checkpointed(ec, *initial(u), mu0, dts, every=every)
+    (gu,) = torch.autograd.grad(J, u)
+
+    def forward(uu):
+        y = (*initial(uu), mu0)
+        with torch.no_grad():
+            for d in dts:
+                y = rk4_step_ev(ec, *y, d)
+            return float(kinetic_energy(ec.rc, y[0], y[1]))
+
+    assert abs(forward(u0) - float(J)) <= 1e-14 * abs(float(J))
+    rng = np.random.default_rng(32)
+    h = 1e-6
+    for it in range(4):
+        v = torch.tensor(rng.standard_normal((m.npoin, 3)), dtype=torch.float64, device="cuda")
+        v = v / torch.linalg.norm(v)
+        fd = (forward(u0 + h * v) - forward(u0 - h * v)) / (2 * h)
+        an = float((gu * v).sum())
+        print(f"  rk4 adjoint direction {it}: fd={fd:.12e} adjoint={an:.12e}", flush=True)
+        assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)

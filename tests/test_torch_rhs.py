@@ -50,3 +50,18 @@ def test_torch_rhs_matches_oracle():
     assert rel_l2(rm.numpy(), cm + dm) < 1e-12
     assert rel_l2(rq.numpy(), cq + dq) < 1e-12
     assert rel_l2(re.numpy(), ce + de) < 1e-12
+
+
+def test_torch_smart_visc_matches_oracle():
+    m, g, gas, s = _case()
+    rng = np.random.default_rng(8)
+    Reta = rng.uniform(-1e-2, 1e-2, m.npoin)
+    p = oracle.default_params(Rgas_inf=gas.Rgas)
+    ref_mu, ref_l = oracle.smart_visc_spectral(m.connec, m.lpoin_w, Reta, g["gpvol"], T.wgp, gas.gamma_gas, s["rho"],
+                                               s["u"], s["csound"], s["Tem"], s["eta"], g["helem_l"], g["helem"],
+                                               T.invAtoIJK, p)
+    t = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float64)
+    mu, ml = TR.smart_visc(m.connec, m.lpoin_w, t(Reta), g["gpvol"], T.wgp, gas.gamma_gas, t(s["rho"]), t(s["u"]),
+                           t(s["csound"]), t(s["Tem"]), t(s["eta"]), g["helem_l"], g["helem"], T.invAtoIJK, gas.Rgas)
+    assert rel_l2(ml.numpy(), ref_l) < 1e-13
+    assert rel_l2(mu.numpy(), ref_mu) < 1e-13

@@ -169,3 +169,63 @@ def lattice_from_elemnode(geo, a):
     """(nelem, 64) element-node array in gmsh node order -> lattice"""
     t = torch.as_tensor(np.asarray(a), dtype=TD)
     return t[:, geo.lat.cpu()].reshape(geo.nelem, 4, 4, 4).to(geo.D.device)
+
+
+def filter_matrix(invAtoIJK):
+    """27-point (1/4,1/2,1/4)^3 mirror filter of smart_visc_spectral
+    (mod_entropy_viscosity.f90:105-121, convertIJK of mod_filters.f90:42-51)
+    as a 64x64 matrix over gmsh node order: mu_e = F @ mue_l."""
+    inv = np.asarray(invAtoIJK) - 1
+    conv = [3, 1, 3, 4, 2, 3]
+    w = [0.25, 0.5, 0.25]
+    F = np.zeros((64, 64))
+    for ii in range(1, 5):
+        for jj in range(1, 5):
+            for kk in range(1, 5):
+                out = inv[conv[ii] - 1, conv[jj] - 1, conv[kk] - 1]
+                for l in (-1, 0, 1):
+                    for m in (-1, 0, 1):
+                        for n in (-1, 0, 1):
+                            a = inv[conv[ii + l] - 1, conv[jj + m] - 1, conv[kk + n] - 1]
+                            F[out, a] += w[l + 1] * w[m + 1] * w[n + 1]
+    return torch.as_tensor(F, dtype=TD)
+
+
+def smart_visc(connec, lpoin_w, Reta, gpvol, wgp, gamma, rho, u, csound, Tem, eta, helem_l, helem, invAtoIJK,
+               Rgas, ce=0.1, cmax=0.5, cglob=1.0, normalise=True):
+    """smart_visc_spectral (mod_entropy_viscosity.f90:14-122) on torch tensors;
+    max/min are taken with first-index subgradients (torch.max over a dim)."""
+    c0 = torch.as_tensor(np.asarray(connec, dtype=np.int64) - 1)
+    lw = torch.as_tensor(np.asarray(lpoin_w, dtype=np.int64) - 1)
+    ne = c0.shape[0]
+    if normalise:
+        ew = eta[lw]
+        mean = ew.sum() / lw.shape[0]
+        norm, _ = torch.max(torch.abs(ew - mean), dim=0)
+    else:
+        norm = torch.tensor(1.0, dtype=TD)
+    gv = torch.as_tensor(np.asarray(gpvol).reshape(64, ne).T, dtype=TD)
+    J = gv / torch.as_tensor(np.asarray(wgp), dtype=TD)[None, :]
+    minJe = torch.clamp(J, max=1000000.0).min(dim=1).values
+    maxJe = torch.clamp(J, min=0.0).max(dim=1).values
+    ug = u[c0]
+    s2 = (ug * ug).sum(-1)
+    vmag = torch.where(s2 > 0, torch.sqrt(torch.where(s2 > 0, s2, torch.ones_like(s2))), torch.zeros_like(s2))
+    maxV, _ = torch.max(vmag, dim=1)
+    maxC, _ = torch.max(csound[c0], dim=1)
+    M = maxV / maxC
+    th = torch.tanh(M ** 15 * np.pi)
+    ceM = torch.where(th > ce, th, torch.full_like(th, ce))
+    rj = minJe / maxJe
+    ced_geo = torch.clamp(1.0 - rj * rj, min=ce)
+    ced = torch.where(ceM > ced_geo, ceM, ced_geo)
+    aux3 = torch.sqrt(Rgas * gamma * Tem[c0])
+    he = torch.as_tensor(np.asarray(helem), dtype=TD)
+    beta = (rho[c0] * he[:, None]) * (cmax / 3.0) * (vmag + aux3)
+    betae, _ = torch.max(beta, dim=1)
+    R1 = rho[c0] * torch.abs(Reta[c0]) / norm
+    hl = torch.as_tensor(np.asarray(helem_l), dtype=TD)
+    Ve = ced[:, None] * R1 * (hl * hl)
+    mue_l = cglob * torch.where(Ve < betae[:, None], Ve, betae[:, None].expand_as(Ve))
+    F = filter_matrix(invAtoIJK)
+    return mue_l @ F.T, mue_l
