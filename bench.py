@@ -67,6 +67,53 @@ def cpu_baseline(seconds: float):
                       f"in {el:.1f} s on 1 host core"}
 
 
+def measured_traffic(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy
+    kernel of known size), when one exists for this workload."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_tgv{n}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def adjoint_ratio(sol, m, reps: int = 5):
+    """Time of one adjoint (VJP) of the full stage RHS over one forward RHS,
+    both through the C ABI on the solver's current state (HIP events on the
+    library stream)."""
+    import torch
+    from sod2d_refmap_amd import _lib
+    g = sol.gas
+    n, ne = m.npoin, m.nelem
+    z = lambda k: torch.zeros(k, dtype=torch.float64, device="cuda")
+    rho, q, E = sol.level("rho"), sol.level("q"), sol.level("E")
+    st = [sol.connec.data_ptr(), sol.lpoin_w.data_ptr(), sol.He.data_ptr(), sol.gpvol.data_ptr(), sol.Ml.data_ptr()]
+    common = st + [rho.data_ptr(), q.data_ptr(), E.data_ptr(), sol.mu_factor.data_ptr(), sol.mu_e.data_ptr(),
+                   sol.mu_sgs.data_ptr(), g.Rgas, g.gamma_gas, g.Cp, g.Prt]
+    outs = [z(n), z(3 * n), z(n), z(n)]
+    lam = [torch.randn(k * n, dtype=torch.float64, device="cuda") for k in (1, 3, 1, 1)]
+    gout = [z(n), z(3 * n), z(n), z(64 * ne)]
+    fwd = lambda: _lib.call("sod_full_rhs", ne, n, m.npoin_w, *common, *[o.data_ptr() for o in outs])
+    adj = lambda: _lib.call("sod_full_rhs_vjp", ne, n, m.npoin_w, *common, *[x.data_ptr() for x in lam],
+                            *[o.data_ptr() for o in gout])
+    fwd()
+    adj()
+    _lib.call("sod_profile_enable", 1)
+    _lib.call("sod_profile_reset")
+    for _ in range(reps):
+        fwd()
+        adj()
+    torch.cuda.synchronize()
+    f_ms, f_n = kernel_ms("full_rhs")
+    a_ms, a_n = kernel_ms("full_rhs_vjp")
+    _lib.call("sod_profile_enable", 0)
+    return {"forward_rhs_ms": round(f_ms / max(f_n, 1), 4), "vjp_ms": round(a_ms / max(a_n, 1), 4),
+            "ratio": round((a_ms / max(a_n, 1)) / (f_ms / max(f_n, 1)), 3) if f_n and a_n else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +122,7 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="elements per direction")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -164,6 +212,24 @@ def main():
                   "alg_GBs": round(stage_gbs, 1), "frac_of_hbm_peak": round(stage_gbs / HBM_PEAK_GBS, 4)},
         "ek_final": ek,
     }
+    traffic = measured_traffic(dom, n)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic
+    # the same steps through the device time loop (graph-captured step, dt on
+    # the device): reported beside the eager figure
+    try:
+        sol.run_device(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol.run_device(K)
+        torch.cuda.synchronize()
+        tl = time.perf_counter() - t0
+        out["time_loop"] = {"ms_per_step": round(tl / K * 1e3, 4),
+                            "GDOF_s": round(5.0 * m.npoin_w * 4 * K / tl / 1e9, 4)}
+    except Exception as exc:  # reported, not fatal: the eager figure stands
+        out["time_loop"] = {"error": str(exc)[:200]}
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_ratio(sol, m)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -763,7 +763,7 @@ extern "C" int sod_full_rhs_vjp(int nelem, int npoin, int npoin_w, const int* co
     const sod_numerical_params& prm = rt().params;
     const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
     if (nelem > 0) {
-        SOD_PROF("stage_rhs_vjp");
+        SOD_PROF("full_rhs_vjp");
         if (lm.has_sgs)
             hipLaunchKernelGGL(k_stage_rhs_vjp<true>, dim3(grid_for(nelem, AWPB)), dim3(64 * AWPB), 0, s, nelem,
                                npoin, lm.conn, lm.He, lm.gpv, lm.mue, lm.musgs, dr, dq, dq + np, dq + 2 * np, dE, dmf,

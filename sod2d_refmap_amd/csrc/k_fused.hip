@@ -847,6 +847,7 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
     SOD_TRY(load_viscosities(Pm, nelem, mu_e, mu_sgs, &has_sgs));
     if (nelem > 0) {
         const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
+        SOD_PROF("full_rhs");
         if (has_sgs)
             hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, npoin,
                                Pm.conn, Pm.He, Pm.gpv, Pm.mue, Pm.musgs, rho, q, q + np, q + 2 * np, E, mu_factor,
