@@ -302,6 +302,28 @@ int sod_time_loop_run(sod_time_loop *loop, int istep0, int nsteps, double cfl_co
                       double *EK, int nEK, int keep_levels);
 int sod_time_loop_destroy(sod_time_loop *loop);
 
+/* A flattened ensemble of ninst independent periodic meshes stepped together
+ * (no reference counterpart: the reference runs one mesh per solver).  The
+ * arrays are those of sod_time_loop_create for the concatenated meshes:
+ * instance i owns elements elem_off[i] .. elem_off[i+1]-1, nodes
+ * node_off[i] .. node_off[i+1]-1 and the lpoin_w entries work_off[i] ..
+ * work_off[i+1]-1 (host arrays of ninst+1 values from 0), and its connec /
+ * lpoin_w hold global node ids (1-based) inside its own node range.  Each
+ * instance has its own dt (CFL condition over its elements) and its own
+ * entropy normalisation; gas constants are shared, mu_factor is per node. */
+int sod_time_loop_create_batched(sod_time_loop **loop, int ninst, const int *elem_off, const int *node_off,
+                                 const int *work_off, int nelem, int npoin, int npoin_w, const int *connec,
+                                 const int *lpoin_w, const double *He, const double *gpvol, const double *Ml,
+                                 const double *helem, const double *helem_l, const double *mu_factor,
+                                 double Rgas, double gamma_gas, double Cp, double Prt, double *rho, double *u,
+                                 double *q, double *pr, double *E, double *Tem, double *csound, double *machno,
+                                 double *e_int, double *eta, double *mu_e, const double *mu_sgs,
+                                 double *mu_fluid, double *mue_l);
+/* nsteps steps of every instance; dt and time are HOST arrays of ninst
+ * values (inout), as in sod_time_loop_run. */
+int sod_time_loop_run_batched(sod_time_loop *loop, int nsteps, double cfl_conv, double cfl_diff, int fixed_dt,
+                              double *dt, double *time, int keep_levels);
+
 #ifdef __cplusplus
 }
 #endif

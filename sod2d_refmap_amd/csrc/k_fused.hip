@@ -283,16 +283,20 @@ constexpr int RED_BLOCKS = 1024;
 
 // Stage s in 0..3 after its element kernel: lumped solve, accumulation,
 // next stage state (s < 3) or final update + EOS (s == 3).
-template <bool FINAL>
+// With BATCHED the nodes belong to independent instances (a flattened
+// ensemble): dt is d_dt[node_inst[n]] and the entropy statistics are formed
+// per instance by their own kernels.
+template <bool FINAL, bool BATCHED>
 __global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, const int* __restrict__ lpoin_w,
                                                       int s, const double* __restrict__ d_dt, double gamma,
-                                                      double Rgas, StepState st) {
+                                                      double Rgas, StepState st,
+                                                      const int* __restrict__ node_inst) {
     __shared__ double s_red[BLK / 64];
     const size_t np = (size_t)npoin;
-    const double dt = *d_dt;
     double eta_acc = 0.0;
     for (int it = blockIdx.x * BLK + threadIdx.x; it < npoin_w; it += gridDim.x * BLK) {
         const int n = lpoin_w[it] - 1;
+        const double dt = BATCHED ? d_dt[node_inst[n]] : d_dt[0];
         const double ml = st.Ml[n];
         double Rf[NRES];
 #pragma unroll
@@ -351,7 +355,7 @@ __global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, co
             st.mu_fluid[n] = sutherland(st.mu_factor[n], T);
         }
     }
-    if (FINAL) {
+    if (FINAL && !BATCHED) {
         // partial sums of eta over working nodes for the normalisation
         double v = eta_acc;
 #pragma unroll
@@ -402,6 +406,53 @@ __global__ void k_set_double_f(double* p, double v) {
     if (threadIdx.x == 0) *p = v;
 }
 
+// Entropy statistics of a flattened ensemble (mod_entropy_viscosity.f90:43-66
+// per instance): the working nodes of instance i are lpoin_w[woff[i] ..
+// woff[i+1]); stats[2i] = sum of eta, stats[2i+1] = max |eta - mean| (as the
+// bits of a non-negative double, ordered like the value).
+__global__ void k_zero_stats(int n, double* __restrict__ stats) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stats[t] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_inst_eta_sum(const int* __restrict__ woff, const int* __restrict__ lpoin_w,
+                                                      const double* __restrict__ eta, double* __restrict__ stats) {
+    __shared__ double s_red[4];
+    const int inst = blockIdx.y;
+    const int b = woff[inst], end = woff[inst + 1];
+    double v = 0.0;
+    for (int t = b + blockIdx.x * 256 + threadIdx.x; t < end; t += gridDim.x * 256) v += eta[lpoin_w[t] - 1];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) unsafeAtomicAdd(&stats[2 * inst], (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]));
+}
+
+__global__ __launch_bounds__(256) void k_inst_eta_dev(const int* __restrict__ woff, const int* __restrict__ lpoin_w,
+                                                      const double* __restrict__ eta, double* __restrict__ stats) {
+    __shared__ double s_red[4];
+    const int inst = blockIdx.y;
+    const int b = woff[inst], end = woff[inst + 1];
+    const double mean = stats[2 * inst] / (double)(end - b);
+    double v = 0.0;
+    for (int t = b + blockIdx.x * 256 + threadIdx.x; t < end; t += gridDim.x * 256)
+        v = fmax(v, fabs(eta[lpoin_w[t] - 1] - mean));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
+        atomicMax(reinterpret_cast<unsigned long long*>(&stats[2 * inst + 1]),
+                  (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// norms[i] = stats[2i + 1] (one value per instance for the entropy kernel)
+__global__ void k_inst_norms(int ninst, const double* __restrict__ stats, double* __restrict__ norms) {
+    for (int t = threadIdx.x; t < ninst; t += blockDim.x) norms[t] = stats[2 * t + 1];
+}
+
 // Tiled transpose of up to two element-node arrays (reference layout
 // (nelem,64), element fastest) into lane layout [e][lane]; 16 elements per
 // block so that both the reads and the writes are contiguous.  `nonzero` is
@@ -447,7 +498,9 @@ __global__ __launch_bounds__(256) void k_smart_visc_lane(
     double gamma_gas, const double* __restrict__ rho, const double* __restrict__ u,
     const double* __restrict__ csound, const double* __restrict__ Tem, const double* __restrict__ d_norm,
     int normalise, double Rgas, double ce, double cmax, double cglob, double* __restrict__ mu_e,
-    double* __restrict__ mue_l, double* __restrict__ mue_lane) {
+    double* __restrict__ mue_l, double* __restrict__ mue_lane, const int* __restrict__ elem_inst) {
+    // elem_inst (flattened ensembles): d_norm holds one normalisation per
+    // instance and element e uses d_norm[elem_inst[e]]
     __shared__ double s_ml[16][65], s_me[16][65];
     __shared__ int s_l2n[64];
     __shared__ double s_w[64];
@@ -465,11 +518,11 @@ __global__ __launch_bounds__(256) void k_smart_visc_lane(
         if (e < nelem) s_me[el][a2] = helem_l[(size_t)e + ne * a2];
     }
     __syncthreads();
-    const double norm = normalise ? d_norm[0] : 1.0;
     const int a = s_l2n[l];
     for (int r = 0; r < 4; ++r) {
         const int el = w + 4 * r, e = e0 + el;
         if (e >= nelem) break;   // uniform per wavefront
+        const double norm = normalise ? d_norm[elem_inst ? elem_inst[e] : 0] : 1.0;
         const size_t eb = (size_t)e * 64 + l;
         const int n = conn_l[eb];
         const double J = gpv_l[eb] / s_w[l];
@@ -726,8 +779,22 @@ struct FusedArgs {
 // read from P's lane-layout copies, dt from device memory.  With
 // `mue_lane_out` the new entropy viscosity is also left in lane layout in P
 // for the next step.
+// A flattened ensemble of independent instances (elements and nodes of
+// instance i contiguous): per-element / per-node instance ids, the offsets of
+// each instance's working nodes in lpoin_w, and device room for its entropy
+// statistics.  ninst == 0 means one plain mesh.
+struct BatchInfo {
+    int ninst = 0;
+    const int* elem_inst = nullptr;
+    const int* node_inst = nullptr;
+    const int* woff = nullptr;
+    double* stats = nullptr;   // 2 * ninst
+    double* norms = nullptr;   // ninst
+    int max_w = 0;             // working nodes of the largest instance
+};
+
 static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_dt, bool has_sgs,
-                               bool mue_lane_out) {
+                               bool mue_lane_out, const BatchInfo* B = nullptr) {
     hipStream_t s = rt().stream;
     const sod_numerical_params& prm = rt().params;
     const int nelem = a.nelem, npoin = a.npoin, npoin_w = a.npoin_w;
@@ -773,12 +840,22 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
                                    st.R, sp);
         }
         SOD_PROF("stage_update");
-        if (stage < 3)
-            hipLaunchKernelGGL(k_stage_update<false>, dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w, a.lpoin_w,
-                               stage, d_dt, a.gamma_gas, a.Rgas, st);
-        else
-            hipLaunchKernelGGL(k_stage_update<true>, dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w, a.lpoin_w,
-                               stage, d_dt, a.gamma_gas, a.Rgas, st);
+        const int* ni = B ? B->node_inst : nullptr;
+        if (stage < 3) {
+            if (B)
+                hipLaunchKernelGGL((k_stage_update<false, true>), dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w,
+                                   a.lpoin_w, stage, d_dt, a.gamma_gas, a.Rgas, st, ni);
+            else
+                hipLaunchKernelGGL((k_stage_update<false, false>), dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w,
+                                   a.lpoin_w, stage, d_dt, a.gamma_gas, a.Rgas, st, ni);
+        } else {
+            if (B)
+                hipLaunchKernelGGL((k_stage_update<true, true>), dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w,
+                                   a.lpoin_w, stage, d_dt, a.gamma_gas, a.Rgas, st, ni);
+            else
+                hipLaunchKernelGGL((k_stage_update<true, false>), dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w,
+                                   a.lpoin_w, stage, d_dt, a.gamma_gas, a.Rgas, st, ni);
+        }
     }
     if (P.nslave > 0)
         hipLaunchKernelGGL(k_slave_sutherland, dim3(grid_for(P.nslave, BLK)), dim3(BLK), 0, s, P.nslave,
@@ -787,12 +864,25 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
     {
         SOD_PROF("smart_visc_fused");
         double* vwork = st.partial + RED_BLOCKS;
-        if (prm.flag_normalise_entropy == 1) launch_eta_norm(npoin_w, a.lpoin_w, a.eta, st.partial, ugrid, vwork);
+        const double* d_norm = vwork + 1;
+        if (prm.flag_normalise_entropy == 1) {
+            if (B) {
+                const dim3 g2(std::max(1, std::min(64, grid_for(B->max_w, 256))), B->ninst);
+                hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(256), 0, s, 2 * B->ninst, B->stats);
+                hipLaunchKernelGGL(k_inst_eta_sum, g2, dim3(256), 0, s, B->woff, a.lpoin_w, a.eta, B->stats);
+                hipLaunchKernelGGL(k_inst_eta_dev, g2, dim3(256), 0, s, B->woff, a.lpoin_w, a.eta, B->stats);
+                hipLaunchKernelGGL(k_inst_norms, dim3(1), dim3(256), 0, s, B->ninst, B->stats, B->norms);
+                d_norm = B->norms;
+            } else {
+                launch_eta_norm(npoin_w, a.lpoin_w, a.eta, st.partial, ugrid, vwork);
+            }
+        }
         if (nelem > 0)
             hipLaunchKernelGGL(k_smart_visc_lane, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, npoin, P.conn,
                                P.gpv, a.helem_l, a.helem, st.aux, a.gamma_gas, a.rho, a.u, a.csound, a.Tem,
-                               vwork + 1, prm.flag_normalise_entropy == 1, prm.Rgas_inf, prm.ce, prm.cmax,
-                               prm.cglob, a.mu_e, a.mue_l, mue_lane_out ? P.mue : nullptr);
+                               d_norm, prm.flag_normalise_entropy == 1, prm.Rgas_inf, prm.ce, prm.cmax,
+                               prm.cglob, a.mu_e, a.mue_l, mue_lane_out ? P.mue : nullptr,
+                               B ? B->elem_inst : nullptr);
     }
 }
 
@@ -891,8 +981,9 @@ __global__ void k_loop_begin(int* __restrict__ ist) {
     if (threadIdx.x == 0) ist[0] += 1;
 }
 
-__global__ void k_loop_time(double* __restrict__ sc) {
-    if (threadIdx.x == 0) sc[1] = sc[1] + sc[0];
+// sc[0..n) = dt of each instance, sc[n..2n) = its time
+__global__ void k_loop_time(double* __restrict__ sc, int n) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) sc[n + t] = sc[n + t] + sc[t];
 }
 
 __device__ __forceinline__ bool is_log_step(int istep, int first, int every) {
@@ -949,6 +1040,10 @@ __global__ __launch_bounds__(256) void k_ek_final_log(int npartial, const double
 void launch_adapt_dt(int nelem, int npoin, const int* connec, const double* helem, const double* u,
                      const double* csound, double cfl_conv, int has_diff, double cfl_diff,
                      const double* mu_fluid, const double* mu_sgs, const double* rho, double* d_dt_bits);
+void launch_adapt_dt_inst(int nelem, int npoin, const int* connec, const double* helem, const double* u,
+                          const double* csound, double cfl_conv, int has_diff, double cfl_diff,
+                          const double* mu_fluid, const double* mu_sgs, const double* rho, const int* elem_inst,
+                          int ninst, double* d_dt_bits);
 
 struct LoopCfg {
     bool valid = false;
@@ -971,8 +1066,13 @@ struct sod_time_loop {
     sod::FusedArgs a;       // level-2 views
     double *rho, *u, *q, *pr, *E, *Tem, *e_int, *eta;   // full level arrays
     double* Ngp = nullptr;  // device copy of the library's Ngp
-    double* sc = nullptr;   // dt, time, scratch
+    double* sc = nullptr;   // dt[ninst], time[ninst]
     int* ist = nullptr;     // istep, EK count
+    int ninst = 1;          // > 1: a flattened ensemble described by B
+    sod::BatchInfo B;
+    int* inst_ids = nullptr;     // elem_inst (nelem) then node_inst (npoin)
+    int* woff = nullptr;         // ninst + 1
+    double* inst_stats = nullptr;   // 3 * ninst
     double* ek_partial = nullptr;
     double* ek_hist = nullptr;
     hipStream_t stream = nullptr;
@@ -1014,7 +1114,7 @@ static void loop_enqueue_step(sod_time_loop* L, const LoopCfg& c) {
         add(L->Tem, 1, 2); add(L->e_int, 1, 2); add(L->eta, 1, 3);
         hipLaunchKernelGGL(k_copy_fields, dim3(cgrid.x, k), dim3(256), 0, s, (long)np, cl);
     }
-    enqueue_fused_step(L->P, a, L->sc, c.has_sgs, true);
+    enqueue_fused_step(L->P, a, L->sc, c.has_sgs, true, L->ninst > 1 ? &L->B : nullptr);
     if (c.keep_levels) {
         // level 3 <- level 1 (CFDSolverBase.f90:1552-1557)
         CopyList cl{};
@@ -1029,16 +1129,21 @@ static void loop_enqueue_step(sod_time_loop* L, const LoopCfg& c) {
         add(L->rho, 1); add(L->E, 1); add(L->q, 3); add(L->eta, 1); add(L->u, 3); add(L->pr, 1);
         hipLaunchKernelGGL(k_copy_fields, dim3(cgrid.x, k), dim3(256), 0, s, (long)np, cl);
     }
-    hipLaunchKernelGGL(k_loop_time, dim3(1), dim3(64), 0, s, L->sc);
+    hipLaunchKernelGGL(k_loop_time, dim3(1), dim3(256), 0, s, L->sc, L->ninst);
     if (c.log_every > 0) {
         hipLaunchKernelGGL(k_ek_partial_log, dim3(RED_BLOCKS), dim3(256), 0, s, a.nelem, a.npoin, a.connec, a.gpvol,
                            L->Ngp, a.rho, a.u, L->ist, c.log_first, c.log_every, L->ek_partial);
         hipLaunchKernelGGL(k_ek_final_log, dim3(1), dim3(256), 0, s, RED_BLOCKS, L->ek_partial, L->ist,
                            c.log_first, c.log_every, c.rho0, c.ek_cap, L->ek_hist);
     }
-    if (!c.fixed_dt)   // evalDt (CFDSolverBase.f90:1132-1141)
-        launch_adapt_dt(a.nelem, a.npoin, a.connec, a.helem, a.u, a.csound, c.cfl_conv, c.has_diff, c.cfl_diff,
-                        a.mu_fluid, a.mu_sgs, a.rho, L->sc);
+    if (!c.fixed_dt) {   // evalDt (CFDSolverBase.f90:1132-1141)
+        if (L->ninst > 1)
+            launch_adapt_dt_inst(a.nelem, a.npoin, a.connec, a.helem, a.u, a.csound, c.cfl_conv, c.has_diff,
+                                 c.cfl_diff, a.mu_fluid, a.mu_sgs, a.rho, L->B.elem_inst, L->ninst, L->sc);
+        else
+            launch_adapt_dt(a.nelem, a.npoin, a.connec, a.helem, a.u, a.csound, c.cfl_conv, c.has_diff,
+                            c.cfl_diff, a.mu_fluid, a.mu_sgs, a.rho, L->sc);
+    }
 }
 
 }  // namespace sod
@@ -1076,13 +1181,12 @@ extern "C" int sod_full_rhs(int nelem, int npoin, int npoin_w, const int* connec
 }
 
 // ------------------------------------------------- whole-step driver (C ABI)
-extern "C" int sod_time_loop_create(sod_time_loop** out, int nelem, int npoin, int npoin_w, const int* connec,
-                                    const int* lpoin_w, const double* He, const double* gpvol, const double* Ml,
-                                    const double* helem, const double* helem_l, const double* mu_factor,
-                                    double Rgas, double gamma_gas, double Cp, double Prt, double* rho, double* u,
-                                    double* q, double* pr, double* E, double* Tem, double* csound,
-                                    double* machno, double* e_int, double* eta, double* mu_e,
-                                    const double* mu_sgs, double* mu_fluid, double* mue_l) {
+static int loop_create(sod_time_loop** out, int nelem, int npoin, int npoin_w, const int* connec,
+                       const int* lpoin_w, const double* He, const double* gpvol, const double* Ml,
+                       const double* helem, const double* helem_l, const double* mu_factor, double Rgas,
+                       double gamma_gas, double Cp, double Prt, double* rho, double* u, double* q, double* pr,
+                       double* E, double* Tem, double* csound, double* machno, double* e_int, double* eta,
+                       double* mu_e, const double* mu_sgs, double* mu_fluid, double* mue_l, int ninst) {
     SOD_TRY(ensure_init());
     if (out == nullptr) return fail(SOD_ERR_ARG, "time_loop_create: out is NULL");
     *out = nullptr;
@@ -1115,6 +1219,7 @@ extern "C" int sod_time_loop_create(sod_time_loop** out, int nelem, int npoin, i
     SOD_TRY(resolve_w(mue_l, "mue_l", &dmuel));
     auto* L = new sod_time_loop();
     const size_t np = (size_t)npoin;
+    L->ninst = ninst;
     L->rho = drho; L->u = du; L->q = dq; L->pr = dpr; L->E = dE; L->Tem = dT; L->e_int = dei; L->eta = deta;
     L->a = FusedArgs{nelem, npoin, npoin_w, dc, dl, dHe, dgv, dMl, dh, dhl, dmf, Rgas, gamma_gas, Cp, Prt,
                      drho + np, du + 3 * np, dq + 3 * np, dpr + np, dE + np, dT + np, dei + np, deta + np, dcs,
@@ -1122,7 +1227,7 @@ extern "C" int sod_time_loop_create(sod_time_loop** out, int nelem, int npoin, i
     int rc = prepare(L->P, nelem, npoin, npoin_w, dc, dl, dHe, dgv, dMl);
     if (rc == SOD_OK) {
         hipError_t e = hipMalloc(&L->Ngp, sizeof(double) * 64 * 64);
-        if (e == hipSuccess) e = hipMalloc(&L->sc, sizeof(double) * 8);
+        if (e == hipSuccess) e = hipMalloc(&L->sc, sizeof(double) * (2 * (size_t)ninst + 8));
         if (e == hipSuccess) e = hipMalloc(&L->ist, sizeof(int) * 4);
         if (e == hipSuccess) e = hipMalloc(&L->ek_partial, sizeof(double) * RED_BLOCKS);
         if (e == hipSuccess)
@@ -1135,15 +1240,79 @@ extern "C" int sod_time_loop_create(sod_time_loop** out, int nelem, int npoin, i
         return rc;
     }
     *out = L;
+    return SOD_OK;
+}
+
+extern "C" int sod_time_loop_create(sod_time_loop** out, int nelem, int npoin, int npoin_w, const int* connec,
+                                    const int* lpoin_w, const double* He, const double* gpvol, const double* Ml,
+                                    const double* helem, const double* helem_l, const double* mu_factor,
+                                    double Rgas, double gamma_gas, double Cp, double Prt, double* rho, double* u,
+                                    double* q, double* pr, double* E, double* Tem, double* csound,
+                                    double* machno, double* e_int, double* eta, double* mu_e,
+                                    const double* mu_sgs, double* mu_fluid, double* mue_l) {
+    SOD_TRY(loop_create(out, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml, helem, helem_l, mu_factor, Rgas,
+                        gamma_gas, Cp, Prt, rho, u, q, pr, E, Tem, csound, machno, e_int, eta, mu_e, mu_sgs,
+                        mu_fluid, mue_l, 1));
     return finish_call();
 }
 
-extern "C" int sod_time_loop_run(sod_time_loop* L, int istep0, int nsteps, double cfl_conv, double cfl_diff,
-                                 int fixed_dt, double* dt, double* time, int log_first, int log_every,
-                                 double rho0, double* EK, int nEK, int keep_levels) {
+extern "C" int sod_time_loop_create_batched(sod_time_loop** out, int ninst, const int* elem_off,
+                                            const int* node_off, const int* work_off, int nelem, int npoin,
+                                            int npoin_w, const int* connec, const int* lpoin_w, const double* He,
+                                            const double* gpvol, const double* Ml, const double* helem,
+                                            const double* helem_l, const double* mu_factor, double Rgas,
+                                            double gamma_gas, double Cp, double Prt, double* rho, double* u,
+                                            double* q, double* pr, double* E, double* Tem, double* csound,
+                                            double* machno, double* e_int, double* eta, double* mu_e,
+                                            const double* mu_sgs, double* mu_fluid, double* mue_l) {
+    if (ninst < 1 || !elem_off || !node_off || !work_off)
+        return fail(SOD_ERR_ARG, "time_loop_create_batched: ninst >= 1 and the offset arrays are required");
+    if (elem_off[0] != 0 || node_off[0] != 0 || work_off[0] != 0 || elem_off[ninst] != nelem ||
+        node_off[ninst] != npoin || work_off[ninst] != npoin_w)
+        return fail(SOD_ERR_ARG, "time_loop_create_batched: offsets must run from 0 to nelem/npoin/npoin_w");
+    int max_w = 0;
+    for (int i = 0; i < ninst; ++i) {
+        if (elem_off[i + 1] <= elem_off[i] || node_off[i + 1] <= node_off[i] || work_off[i + 1] <= work_off[i])
+            return fail(SOD_ERR_ARG, "time_loop_create_batched: instance %d is empty or the offsets decrease", i);
+        max_w = std::max(max_w, work_off[i + 1] - work_off[i]);
+    }
+    SOD_TRY(loop_create(out, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml, helem, helem_l, mu_factor, Rgas,
+                        gamma_gas, Cp, Prt, rho, u, q, pr, E, Tem, csound, machno, e_int, eta, mu_e, mu_sgs,
+                        mu_fluid, mue_l, ninst));
+    sod_time_loop* L = *out;
+    std::vector<int> ids((size_t)nelem + (size_t)npoin);
+    for (int i = 0; i < ninst; ++i) {
+        for (int e = elem_off[i]; e < elem_off[i + 1]; ++e) ids[e] = i;
+        for (int n = node_off[i]; n < node_off[i + 1]; ++n) ids[(size_t)nelem + n] = i;
+    }
+    hipError_t e = hipMalloc(&L->inst_ids, sizeof(int) * ids.size());
+    if (e == hipSuccess) e = hipMalloc(&L->woff, sizeof(int) * (ninst + 1));
+    if (e == hipSuccess) e = hipMalloc(&L->inst_stats, sizeof(double) * 3 * (size_t)ninst);
+    if (e == hipSuccess) e = hipMemcpy(L->inst_ids, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(L->woff, work_off, sizeof(int) * (ninst + 1), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        sod_time_loop_destroy(L);
+        *out = nullptr;
+        return fail(SOD_ERR_HIP, "time_loop_create_batched: %s", hipGetErrorString(e));
+    }
+    L->B.ninst = ninst;
+    L->B.elem_inst = L->inst_ids;
+    L->B.node_inst = L->inst_ids + nelem;
+    L->B.woff = L->woff;
+    L->B.stats = L->inst_stats;
+    L->B.norms = L->inst_stats + 2 * ninst;
+    L->B.max_w = max_w;
+    return finish_call();
+}
+
+static int loop_run(sod_time_loop* L, int istep0, int nsteps, double cfl_conv, double cfl_diff, int fixed_dt,
+                    double* dt, double* time, int log_first, int log_every, double rho0, double* EK, int nEK,
+                    int keep_levels) {
     SOD_TRY(ensure_init());
     if (L == nullptr || dt == nullptr || time == nullptr) return fail(SOD_ERR_ARG, "time_loop_run: NULL argument");
     if (nsteps < 0) return fail(SOD_ERR_ARG, "time_loop_run: nsteps < 0");
+    const int ni = L->ninst;
+    if (ni > 1 && log_every > 0) return fail(SOD_ERR_ARG, "time_loop_run: EK logging is per single mesh");
     int nlog = 0;
     if (log_every > 0)
         for (int i = istep0; i < istep0 + nsteps; ++i) nlog += (i >= log_first && (i - log_first) % log_every == 0);
@@ -1171,10 +1340,14 @@ extern "C" int sod_time_loop_run(sod_time_loop* L, int istep0, int nsteps, doubl
         L->ek_hist = nullptr;
         SOD_HIP(hipMalloc(&L->ek_hist, sizeof(double) * c.ek_cap));
     }
-    const double hsc[2] = {*dt, *time};
+    std::vector<double> hsc(2 * (size_t)ni);
+    for (int i = 0; i < ni; ++i) {
+        hsc[i] = dt[i];
+        hsc[ni + i] = time[i];
+    }
     const int hist[2] = {istep0 - 1, 0};
     hipStream_t saved = rt().stream;
-    SOD_HIP(hipMemcpyAsync(L->sc, hsc, sizeof(hsc), hipMemcpyHostToDevice, saved));
+    SOD_HIP(hipMemcpyAsync(L->sc, hsc.data(), sizeof(double) * hsc.size(), hipMemcpyHostToDevice, saved));
     SOD_HIP(hipMemcpyAsync(L->ist, hist, sizeof(hist), hipMemcpyHostToDevice, saved));
     SOD_HIP(hipStreamSynchronize(saved));
     rt().stream = L->stream;
@@ -1206,12 +1379,26 @@ extern "C" int sod_time_loop_run(sod_time_loop* L, int istep0, int nsteps, doubl
     rt().stream = saved;
     prof_set(prof);
     SOD_TRY(rc);
-    double osc[2];
-    SOD_HIP(hipMemcpy(osc, L->sc, sizeof(osc), hipMemcpyDeviceToHost));
-    *dt = osc[0];
-    *time = osc[1];
+    SOD_HIP(hipMemcpy(hsc.data(), L->sc, sizeof(double) * hsc.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < ni; ++i) {
+        dt[i] = hsc[i];
+        time[i] = hsc[ni + i];
+    }
     if (nlog > 0) SOD_HIP(hipMemcpy(EK, L->ek_hist, sizeof(double) * nlog, hipMemcpyDeviceToHost));
     return SOD_OK;
+}
+
+extern "C" int sod_time_loop_run(sod_time_loop* L, int istep0, int nsteps, double cfl_conv, double cfl_diff,
+                                 int fixed_dt, double* dt, double* time, int log_first, int log_every,
+                                 double rho0, double* EK, int nEK, int keep_levels) {
+    if (L && L->ninst != 1) return fail(SOD_ERR_ARG, "time_loop_run: use sod_time_loop_run_batched for an ensemble");
+    return loop_run(L, istep0, nsteps, cfl_conv, cfl_diff, fixed_dt, dt, time, log_first, log_every, rho0, EK, nEK,
+                    keep_levels);
+}
+
+extern "C" int sod_time_loop_run_batched(sod_time_loop* L, int nsteps, double cfl_conv, double cfl_diff,
+                                         int fixed_dt, double* dt, double* time, int keep_levels) {
+    return loop_run(L, 1, nsteps, cfl_conv, cfl_diff, fixed_dt, dt, time, 1, 0, 1.0, nullptr, 0, keep_levels);
 }
 
 extern "C" int sod_time_loop_destroy(sod_time_loop* L) {
@@ -1224,6 +1411,9 @@ extern "C" int sod_time_loop_destroy(sod_time_loop* L) {
     (void)hipFree(L->ist);
     (void)hipFree(L->ek_partial);
     (void)hipFree(L->ek_hist);
+    (void)hipFree(L->inst_ids);
+    (void)hipFree(L->woff);
+    (void)hipFree(L->inst_stats);
     L->P.release();
     delete L;
     return SOD_OK;

@@ -242,6 +242,80 @@ __global__ __launch_bounds__(256) void k_adapt_dt(int nelem, int npoin, const in
     }
 }
 
+// adapt_dt_cfl of each instance of a flattened ensemble: element e belongs to
+// instance elem_inst[e] and its dt folds into out_bits[elem_inst[e]].  Same
+// per-element arithmetic as k_adapt_dt; a tile whose 16 elements share one
+// instance folds its minimum with one atomic.
+__global__ __launch_bounds__(256) void k_adapt_dt_inst(int nelem, int npoin, const int* __restrict__ connec,
+                                                       const double* __restrict__ helem,
+                                                       const double* __restrict__ u,
+                                                       const double* __restrict__ csound, double cfl_conv,
+                                                       int has_diff, double cfl_diff,
+                                                       const double* __restrict__ mu_fluid,
+                                                       const double* __restrict__ mu_sgs,
+                                                       const double* __restrict__ rho,
+                                                       const int* __restrict__ elem_inst,
+                                                       unsigned long long* __restrict__ out_bits) {
+    __shared__ double s_L[4][16], s_M[4][16];
+    const size_t np = (size_t)npoin, ne = (size_t)nelem;
+    const int el = threadIdx.x & 15, a0 = threadIdx.x >> 4;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ntiles = (nelem + 15) / 16;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int e = t * 16 + el;
+        double L3 = 0.0, mx = 0.0;
+        if (e < nelem) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const size_t ea = (size_t)e + ne * (a0 + 16 * r);
+                const int n = connec[ea] - 1;
+                double umag = fabs(u[n]);
+                umag = fmax(umag, fabs(u[n + np]));
+                umag = fmax(umag, fabs(u[n + 2 * np]));
+                L3 = fmax(L3, umag + csound[n]);
+                if (has_diff) mx = fmax(mx, (rho[n] * mu_sgs[ea] + mu_fluid[n]) / rho[n]);
+            }
+        }
+        L3 = fmax(L3, __shfl_xor(L3, 16));
+        L3 = fmax(L3, __shfl_xor(L3, 32));
+        mx = fmax(mx, __shfl_xor(mx, 16));
+        mx = fmax(mx, __shfl_xor(mx, 32));
+        if (lane < 16) {
+            s_L[w][lane] = L3;
+            s_M[w][lane] = mx;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            double dt_l = HUGE_VAL;
+            int inst = -1;
+            if (threadIdx.x < 16 && e < nelem) {
+                L3 = fmax(fmax(s_L[0][el], s_L[1][el]), fmax(s_L[2][el], s_L[3][el]));
+                mx = fmax(fmax(s_M[0][el], s_M[1][el]), fmax(s_M[2][el], s_M[3][el]));
+                const double hh = helem[e] / 7.0;
+                dt_l = cfl_conv * hh / L3;
+                if (has_diff) dt_l = fmin(dt_l, cfl_diff * (hh * hh) / mx);
+                inst = elem_inst[e];
+            }
+            const int last = min(t * 16 + 15, nelem - 1);
+            const bool one = elem_inst[t * 16] == elem_inst[last];
+            if (one) {
+                double v = dt_l;
+#pragma unroll
+                for (int o = 8; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+                if (threadIdx.x == 0 && v < HUGE_VAL)
+                    atomicMin(out_bits + elem_inst[t * 16], (unsigned long long)__double_as_longlong(v));
+            } else if (inst >= 0) {
+                atomicMin(out_bits + inst, (unsigned long long)__double_as_longlong(dt_l));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_set_doubles(int n, double* p, double v) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) p[t] = v;
+}
+
 // ------------------------------------------------------------- EK
 // volAvg_EK, mod_analysis.f90:10-44 (Ngp is the identity for collocated
 // GLL nodes, so only the diagonal of Ngp is used).
@@ -467,6 +541,18 @@ void launch_adapt_dt(int nelem, int npoin, const int* connec, const double* hele
     if (nelem > 0)
         hipLaunchKernelGGL(k_adapt_dt, dim3(std::min(2048, grid_for(nelem, 16))), dim3(256), 0, s, nelem, npoin, connec,
                            helem, u, csound, cfl_conv, has_diff, cfl_diff, mu_fluid, mu_sgs, rho,
+                           reinterpret_cast<unsigned long long*>(d_dt_bits));
+}
+
+void launch_adapt_dt_inst(int nelem, int npoin, const int* connec, const double* helem, const double* u,
+                          const double* csound, double cfl_conv, int has_diff, double cfl_diff,
+                          const double* mu_fluid, const double* mu_sgs, const double* rho, const int* elem_inst,
+                          int ninst, double* d_dt_bits) {
+    hipStream_t s = rt().stream;
+    hipLaunchKernelGGL(k_set_doubles, dim3(1), dim3(256), 0, s, ninst, d_dt_bits, 100000000000000.0);
+    if (nelem > 0)
+        hipLaunchKernelGGL(k_adapt_dt_inst, dim3(std::min(2048, grid_for(nelem, 16))), dim3(256), 0, s, nelem, npoin,
+                           connec, helem, u, csound, cfl_conv, has_diff, cfl_diff, mu_fluid, mu_sgs, rho, elem_inst,
                            reinterpret_cast<unsigned long long*>(d_dt_bits));
 }
 
