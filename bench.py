@@ -114,6 +114,45 @@ def adjoint_ratio(sol, m, reps: int = 5):
             "ratio": round((a_ms / max(a_n, 1)) / (f_ms / max(f_n, 1)), 3) if f_n and a_n else None}
 
 
+def batch_speedup(count: int, steps: int):
+    """The seeded ensemble of `count` TGV boxes (n^3, n in {8,12,16,24,32},
+    Re ~ U[400,3200], Mach ~ U[0.05,0.3], 1e-2 Fourier perturbation): `steps`
+    steps of the batched device loop against the same steps taken member by
+    member through the plain entry points, and the largest relative L2
+    difference between the two results."""
+    import torch
+    from sod2d_refmap_amd.ensemble import Ensemble, ensemble_members, sequential_solvers, sequential_step
+    t0 = time.perf_counter()
+    members = ensemble_members(count)
+    E = Ensemble(members)
+    sols = sequential_solvers(members)
+    log(f"ensemble of {count}: {E.nelem} elements, {E.npoin_w} working nodes "
+        f"(setup {time.perf_counter() - t0:.1f}s)")
+    E.run(1)
+    sequential_step(sols)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E.run(steps)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sequential_step(sols)
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t0
+    import numpy as np
+    err = 0.0
+    for i, sol in enumerate(sols):
+        a, b = E.member_state(i), sol.state()
+        for k in ("rho", "q", "E", "mu_e"):
+            err = max(err, float(np.linalg.norm(a[k] - b[k]) / max(np.linalg.norm(b[k]), 1e-300)))
+    E.close()
+    return {"members": count, "elements": E.nelem, "working_nodes": E.npoin_w, "steps": steps,
+            "batched_ms_per_step": round(tb / steps * 1e3, 3), "sequential_ms_per_step": round(ts / steps * 1e3, 3),
+            "speedup": round(ts / tb, 3), "batched_GDOF_s": round(5.0 * E.npoin_w * 4 * steps / tb / 1e9, 4),
+            "max_rel_l2_vs_sequential": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +162,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batch", type=int, default=0, help="also time an ensemble of this many boxes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -230,6 +270,8 @@ def main():
         out["time_loop"] = {"error": str(exc)[:200]}
     if not args.no_adjoint:
         out["adjoint"] = adjoint_ratio(sol, m)
+    if args.batch > 0:
+        out["batch"] = batch_speedup(args.batch, max(2, K // 2))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -72,11 +72,11 @@ _SIGS = {
     "sod_smart_visc_spectral_vjp": [c_int] * 3 + [P] * 4 + [c_double] + [P] * 14,
     "sod_rk_4_main": ([c_int] * 7 + [P] * 9 + [c_int] + [P] * 8 + [c_double] + [P] * 2 +
                       [c_double] * 4 + [P] * 22),
-    "sod_time_loop_create": [ctypes.POINTER(c_void_p)] + [c_int] * 3 + [P] * 9 + [c_double] * 4 + [P] * 14,
+    "sod_time_loop_create": [ctypes.POINTER(c_void_p)] + [c_int] * 3 + [P] * 8 + [c_double] * 4 + [P] * 14,
     "sod_time_loop_run": [c_void_p, c_int, c_int, c_double, c_double, c_int, ctypes.POINTER(c_double),
                           ctypes.POINTER(c_double), c_int, c_int, c_double, ctypes.POINTER(c_double), c_int, c_int],
     "sod_time_loop_destroy": [c_void_p],
-    "sod_time_loop_create_batched": ([ctypes.POINTER(c_void_p), c_int, P, P, P] + [c_int] * 3 + [P] * 9 +
+    "sod_time_loop_create_batched": ([ctypes.POINTER(c_void_p), c_int, P, P, P] + [c_int] * 3 + [P] * 8 +
                                      [c_double] * 4 + [P] * 14),
     "sod_time_loop_run_batched": [c_void_p, c_int, c_double, c_double, c_int, P, P, c_int],
 }

@@ -248,8 +248,8 @@ class _Checkpointed(torch.autograd.Function):
         with torch.enable_grad():
             yf = tuple(t.detach().requires_grad_(True) for t in y[:3])
             J = obj(*yf)
-            lam = torch.autograd.grad(J, yf)
-        ctx.lam = lam
+            lam = torch.autograd.grad(J, yf, allow_unused=True)
+        ctx.lam = tuple(torch.zeros_like(t) if g is None else g for t, g in zip(yf, lam))
         return J.detach()
 
     @staticmethod

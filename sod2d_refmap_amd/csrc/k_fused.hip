@@ -41,8 +41,10 @@ constexpr int BLK = 256;
 // 1..3, from LDS); the derivative coefficients are tabulated by (own index,
 // offset): sDn[o][d] = D[o][(o+d)&3] for the strong-form derivative and
 // sDt[o][d] = D[(o+d)&3][o] for the weak-form divergence.
+// Three wavefronts per SIMD (amdgpu_waves_per_eu(3)) measured faster than
+// the two the unconstrained register allocation gives, despite a few spills.
 template <bool HAS_SGS>
-__global__ __launch_bounds__(64 * WPB) void k_stage_rhs(
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_stage_rhs(
     int nelem, int npoin, const int* __restrict__ conn, const double* __restrict__ He,
     const double* __restrict__ gpv, const double* __restrict__ mue, const double* __restrict__ musgs,
     const double* __restrict__ rho, const double* __restrict__ q0, const double* __restrict__ q1,
@@ -62,7 +64,16 @@ __global__ __launch_bounds__(64 * WPB) void k_stage_rhs(
     }
     __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int e = blockIdx.x * WPB + w;
+    // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
+    // on XCD b % 8); give XCD x the contiguous range of element blocks
+    // [x*q + min(x, r8), ...) so that neighbouring elements, which share
+    // nodes, meet in the same L2 for their gathers and atomics.
+    int lb;
+    {
+        const int nb = gridDim.x, b = blockIdx.x, q = nb >> 3, r8 = nb & 7, x = b & 7;
+        lb = x * q + min(x, r8) + (b >> 3);
+    }
+    const int e = lb * WPB + w;
     if (e >= nelem) return;   // uniform per wavefront; only wave-level sync below
     const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
     const size_t eb = (size_t)e * 64;

@@ -49,3 +49,19 @@ def test_errors_without_init():
     lib = _lib.lib()
     rc = lib.sod_init(4)
     assert rc == -4 and b"porder" in lib.sod_last_error()
+
+
+def test_binding_argument_counts_match_header():
+    """Every ctypes signature has as many arguments as the C prototype."""
+    txt = ""
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt += re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+    bad = []
+    for name, args in _lib._SIGS.items():
+        m = re.search(r"\b" + name + r"\s*\(([^;]*?)\)\s*;", txt, re.S)
+        assert m, name
+        params = m.group(1).strip()
+        n = 0 if params in ("void", "") else len(params.split(","))
+        if n != len(args):
+            bad.append((name, n, len(args)))
+    assert not bad, bad
