@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batch", type=int, default=0, help="also time an ensemble of this many boxes")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="start with one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -175,6 +177,13 @@ def main():
     from sod2d_refmap_amd.solver import PeriodicSolver
 
     n = args.n
+    if args.copy_probe:
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+        log("copy probe: 1 GiB device-to-device")
     t0 = time.perf_counter()
     m = mesh.periodic_box(n)
     gas = cases.GasParams()

@@ -191,18 +191,23 @@ def test_rk4_chain_matches_rk_4_main(gpu):
         assert err < 1e-12, (name, err)
 
 
-def test_rk4_adjoint_checkpointed_fd(gpu):
+@pytest.mark.parametrize("normalise,tol", [(0, 1e-6), (1, 1e-5)], ids=["norm_off", "norm_on"])
+def test_rk4_adjoint_checkpointed_fd(gpu, normalise, tol):
     """Gradient of the kinetic energy after K RK4 steps with respect to the
     initial velocity (at fixed density and pressure), through the hand-written
     adjoints with checkpointing, against central differences of the forward
-    chain (same dt sequence) along seeded random unit directions."""
+    chain (same dt sequence) along seeded random unit directions.  With the
+    entropy normalisation on, the global max |eta - mean| is a subgradient
+    whose maximiser can switch inside the difference stencil when two nodes
+    are nearly tied (the Taylor-Green state is symmetric), so the difference
+    quotient is compared at 1e-5 there and at 1e-6 without it."""
     from sod2d_refmap_amd.autograd import kinetic_energy, rk4_step_ev, run_checkpointed
     from sod2d_refmap_amd.device import numerical_params, set_numerical_params
     m = pmesh.periodic_box(2, curved_amp=0.05, perm_seed=6)
     g = mesh_geometry(m)
     gas = cases.GasParams()
     s = cases.tgv_initial_state(m, gas, perturb_seed=8, perturb_amp=5e-2)
-    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas, flag_normalise_entropy=normalise))
     ec = _ev_context(m, g, gas, s)
     rho0, p0, u0 = _cuda(s["rho"]), _cuda(s["pr"]), _cuda(s["u"])
     g1 = gas.gamma_gas - 1.0
@@ -235,4 +240,5 @@ def test_rk4_adjoint_checkpointed_fd(gpu):
         fd = (forward(u0 + h * v) - forward(u0 - h * v)) / (2 * h)
         an = float((gu * v).sum())
         print(f"  rk4 adjoint direction {it}: fd={fd:.12e} adjoint={an:.12e}", flush=True)
-        assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+        assert abs(fd - an) <= tol * abs(an), (fd, an)
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
