@@ -68,11 +68,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))
     // on XCD b % 8); give XCD x the contiguous range of element blocks
     // [x*q + min(x, r8), ...) so that neighbouring elements, which share
     // nodes, meet in the same L2 for their gathers and atomics.
-    int lb;
-    {
-        const int nb = gridDim.x, b = blockIdx.x, q = nb >> 3, r8 = nb & 7, x = b & 7;
-        lb = x * q + min(x, r8) + (b >> 3);
-    }
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * WPB + w;
     if (e >= nelem) return;   // uniform per wavefront; only wave-level sync below
     const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
@@ -474,7 +470,7 @@ __global__ __launch_bounds__(256) void k_to_lane_tiled(int nelem, const double* 
     __shared__ double sA[16][65], sB[16][65];
     __shared__ int s_l2n[64];
     if (threadIdx.x < 64) s_l2n[threadIdx.x] = c_tab.lane2node[threadIdx.x];
-    const int e0 = blockIdx.x * 16;
+    const int e0 = xcd_block(blockIdx.x, gridDim.x) * 16;
     const size_t ne = (size_t)nelem;
     int nz = 0;
     for (int idx = threadIdx.x; idx < 1024; idx += 256) {
@@ -521,7 +517,7 @@ __global__ __launch_bounds__(256) void k_smart_visc_lane(
         s_w[threadIdx.x] = c_tab.wgp[a];
     }
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int e0 = blockIdx.x * 16;
+    const int e0 = xcd_block(blockIdx.x, gridDim.x) * 16;
     const size_t np = (size_t)npoin, ne = (size_t)nelem;
     // helem_l(nelem,64) of the block's elements, read contiguously into s_me
     for (int idx = threadIdx.x; idx < 1024; idx += 256) {

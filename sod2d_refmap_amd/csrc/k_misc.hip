@@ -5,7 +5,7 @@
 #include <algorithm>
 #include <vector>
 
-#include "sod2d_internal.hpp"
+#include "stage_common.hpp"
 
 namespace sod {
 
@@ -22,17 +22,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
 template <int OP>   // 0 sum, 1 max
 __device__ __forceinline__ double block_reduce(double v, double* s_tmp) {
     v = OP == 0 ? wave_sum(v) : wave_max(v);
@@ -199,9 +188,13 @@ __global__ __launch_bounds__(256) void k_adapt_dt(int nelem, int npoin, const in
     const size_t np = (size_t)npoin, ne = (size_t)nelem;
     const int el = threadIdx.x & 15, a0 = threadIdx.x >> 4;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // each block takes a contiguous chunk of tiles; blocks of one XCD
+    // take neighbouring chunks (shared nodes stay in that XCD's L2)
     const int ntiles = (nelem + 15) / 16;
+    const int chunk = (ntiles + gridDim.x - 1) / gridDim.x;
+    const int t0 = xcd_block(blockIdx.x, gridDim.x) * chunk, t1 = min(ntiles, t0 + chunk);
     double best = HUGE_VAL;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = t0; t < t1; ++t) {
         const int e = t * 16 + el;
         double L3 = 0.0, mx = 0.0;
         if (e < nelem) {

@@ -24,6 +24,15 @@ __device__ __forceinline__ double wave_min(double v) {
     return v;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8).  Returns the logical block index that gives XCD x the
+// contiguous range [x*q + min(x, r), ...) of the nb blocks (a bijection), so
+// that blocks working on neighbouring elements share an L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb >> 3, r8 = nb & 7, x = b & 7;
+    return x * q + min(x, r8) + (b >> 3);
+}
+
 // orders LDS accesses of the lanes of one wavefront
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -13,12 +13,16 @@ KEYS = ("rho", "q", "E", "u", "pr", "Tem", "eta", "csound", "machno", "mu_fluid"
 
 
 def _members():
-    specs = [(2, 0.0, None, 800.0, 0.2, 1), (3, 0.05, 9, 1600.0, 0.1, 2), (2, 0.0, None, 3000.0, 0.05, 3)]
+    # (n, curved_amp, perm_seed, Re, Mach, perturb_seed, perturb_amp). The
+    # 2^3 box at Mach 0.2 takes a 1% perturbation: with 5% the scheme itself
+    # (the oracle included) diverges there on the second step.
+    specs = [(2, 0.0, None, 800.0, 0.2, 1, 1e-2), (3, 0.05, 9, 1600.0, 0.1, 2, 5e-2),
+             (2, 0.0, None, 3000.0, 0.05, 3, 5e-2)]
     out = []
-    for n, amp, perm, Re, Ma, seed in specs:
+    for n, amp, perm, Re, Ma, seed, pamp in specs:
         m = pmesh.periodic_box(n, curved_amp=amp, perm_seed=perm)
         gas = cases.GasParams(Re=Re, Mach=Ma)
-        out.append((m, gas, cases.tgv_initial_state(m, gas, perturb_seed=seed, perturb_amp=5e-2)))
+        out.append((m, gas, cases.tgv_initial_state(m, gas, perturb_seed=seed, perturb_amp=pamp)))
     return out
 
 
@@ -37,6 +41,7 @@ def test_ensemble_matches_members(gpu):
         for i, s in enumerate(sols):
             got, ref = E.member_state(i), s.state()
             for k in KEYS:
+                assert np.isfinite(got[k]).all(), (i, k)
                 err = rel_l2(got[k], ref[k])
                 assert err < 1e-12, (i, k, err)
             assert abs(E.dt[i] / s.dt - 1) < 1e-12 and abs(E.time[i] / s.time - 1) < 1e-12
