@@ -59,6 +59,29 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
     return LIB
 
 
+CXX = os.environ.get("SOD_CXX", "g++")
+HDF5_ROOT = os.environ.get("SOD_HDF5_ROOT", "/opt/conda")
+IO_LIB = os.path.join(HERE, "libsod2d_io.so")
+
+
+def build_io(verbose: bool = True) -> str:
+    """Compiles the host-only HDF5 mesh/restart library (csrc/io_h5.cpp) into
+    sod2d_refmap_amd/libsod2d_io.so against the HDF5 C library."""
+    inc, lib = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        raise RuntimeError(f"hdf5.h not found under {inc} (set SOD_HDF5_ROOT)")
+    tmp = IO_LIB + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{inc}",
+           os.path.join(CSRC, "io_h5.cpp"), f"-L{lib}", "-lhdf5", f"-Wl,-rpath,{lib}", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"io library build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, IO_LIB)
+    if verbose:
+        print(f"built {IO_LIB}", flush=True)
+    return IO_LIB
+
+
 FC = os.environ.get("SOD_FC", "/opt/rocm/bin/amdflang")
 FORTRAN = os.path.join(HERE, "fortran")
 FBUILD = os.path.join(FORTRAN, "_build")
@@ -91,5 +114,6 @@ def build_fortran(verbose: bool = True) -> str:
 
 if __name__ == "__main__":
     build()
+    build_io()
     build_fortran()
     sys.exit(0)

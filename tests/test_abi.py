@@ -7,14 +7,19 @@ import re
 
 import numpy as np
 
-from sod2d_refmap_amd import _lib
+from sod2d_refmap_amd import _lib, h5io
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
+HEADER_LIB = {"sod2d_hip.h": _lib.LIB_PATH, "sod2d_io.h": h5io.LIB_PATH}
+
+
+def declared_symbols(header: str | None = None):
     names = set()
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        if header is not None and os.path.basename(h) != header:
+            continue
         txt = open(h).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"^\s*(?:const\s+)?(?:int|void|char|double)\s*\**\s*(sod_\w+)\s*\(", txt, re.M):
@@ -28,14 +33,20 @@ def test_header_declares_entry_points():
             "sod_generic_scalar_convec_ijk"} <= names
 
 
+def test_every_header_has_its_library():
+    assert {os.path.basename(h) for h in glob.glob(os.path.join(ROOT, "include", "*.h"))} == set(HEADER_LIB)
+
+
 def test_library_exports_every_declared_symbol():
-    lib = ctypes.CDLL(_lib.LIB_PATH)
-    missing = [n for n in sorted(declared_symbols()) if not hasattr(lib, n)]
-    assert not missing, missing
+    for header, path in HEADER_LIB.items():
+        lib = ctypes.CDLL(path)
+        missing = [n for n in sorted(declared_symbols(header)) if not hasattr(lib, n)]
+        assert not missing, (header, missing)
 
 
 def test_binding_covers_header():
-    assert declared_symbols() <= set(_lib.exported_symbols())
+    assert declared_symbols("sod2d_hip.h") <= set(_lib.exported_symbols())
+    assert declared_symbols("sod2d_io.h") == set(h5io._SIGS)
 
 
 def test_host_tables_without_gpu():
@@ -57,7 +68,7 @@ def test_binding_argument_counts_match_header():
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         txt += re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
     bad = []
-    for name, args in _lib._SIGS.items():
+    for name, args in list(_lib._SIGS.items()) + [(k, v[0]) for k, v in h5io._SIGS.items()]:
         m = re.search(r"\b" + name + r"\s*\(([^;]*?)\)\s*;", txt, re.S)
         assert m, name
         params = m.group(1).strip()
