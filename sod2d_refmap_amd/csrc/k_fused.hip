@@ -284,6 +284,13 @@ struct StepState {
     double *R;      // 6 x npoin
     double *Rsum;   // 6 x npoin
     double *partial;   // reduction partials (RED_BLOCKS)
+    // CFL time step of the next step folded into the final update (node_dt;
+    // one mesh, mu_sgs = 0): per-node h_min, and the bits of the running
+    // minimum.  hmin == nullptr: not folded.
+    const double* hmin;
+    unsigned long long* dt_bits;
+    double cfl_conv, cfl_diff;
+    int has_diff;
 };
 
 constexpr int RED_BLOCKS = 1024;
@@ -299,8 +306,10 @@ __global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, co
                                                       double Rgas, StepState st,
                                                       const int* __restrict__ node_inst) {
     __shared__ double s_red[BLK / 64];
+    __shared__ double s_dt[BLK / 64];
     const size_t np = (size_t)npoin;
     double eta_acc = 0.0;
+    double dt_min = HUGE_VAL;
     for (int it = blockIdx.x * BLK + threadIdx.x; it < npoin_w; it += gridDim.x * BLK) {
         const int n = lpoin_w[it] - 1;
         const double dt = BATCHED ? d_dt[node_inst[n]] : d_dt[0];
@@ -359,7 +368,14 @@ __global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, co
             // entropy residual Reta = -Reta_sum (time_integ.f90:467-471),
             // kept in the (now free) first aux array for the entropy viscosity
             st.aux[n] = -Rs[R_ETA];
-            st.mu_fluid[n] = sutherland(st.mu_factor[n], T);
+            const double mu = sutherland(st.mu_factor[n], T);
+            st.mu_fluid[n] = mu;
+            if (!BATCHED && st.hmin != nullptr) {
+                const double h = st.hmin[n];
+                if (h < HUGE_VAL)
+                    dt_min = fmin(dt_min, node_dt(h, uv[0], uv[1], uv[2], c, mu, r, st.cfl_conv, st.has_diff,
+                                                  st.cfl_diff));
+            }
         }
     }
     if (FINAL && !BATCHED) {
@@ -373,6 +389,16 @@ __global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, co
             double t = 0.0;
             for (int q = 0; q < BLK / 64; ++q) t += s_red[q];
             st.partial[blockIdx.x] = t;
+        }
+        if (st.hmin != nullptr) {
+            const double v = wave_min(dt_min);
+            if ((threadIdx.x & 63) == 0) s_dt[threadIdx.x >> 6] = v;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double m = s_dt[0];
+                for (int q = 1; q < BLK / 64; ++q) m = fmin(m, s_dt[q]);
+                if (m < HUGE_VAL) atomicMin(st.dt_bits, (unsigned long long)__double_as_longlong(m));
+            }
         }
     }
 }
@@ -800,8 +826,17 @@ struct BatchInfo {
     int max_w = 0;             // working nodes of the largest instance
 };
 
+// The CFL time step of the next step folded into the final update (single
+// mesh without sgs viscosity): per-node h_min and where the minimum goes.
+struct DtFold {
+    const double* hmin;
+    double* dt_bits;
+    double cfl_conv, cfl_diff;
+    int has_diff;
+};
+
 static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_dt, bool has_sgs,
-                               bool mue_lane_out, const BatchInfo* B = nullptr) {
+                               bool mue_lane_out, const BatchInfo* B = nullptr, const DtFold* fold = nullptr) {
     hipStream_t s = rt().stream;
     const sod_numerical_params& prm = rt().params;
     const int nelem = a.nelem, npoin = a.npoin, npoin_w = a.npoin_w;
@@ -826,6 +861,17 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
     st.R = P.work + 5 * np;
     st.Rsum = P.work + 11 * np;
     st.partial = P.work + 17 * np;
+    st.hmin = nullptr;
+    st.dt_bits = nullptr;
+    st.cfl_conv = st.cfl_diff = 0.0;
+    st.has_diff = 0;
+    if (fold != nullptr && B == nullptr && !has_sgs) {
+        st.hmin = fold->hmin;
+        st.dt_bits = reinterpret_cast<unsigned long long*>(fold->dt_bits);
+        st.cfl_conv = fold->cfl_conv;
+        st.cfl_diff = fold->cfl_diff;
+        st.has_diff = fold->has_diff;
+    }
     const StageParams sp = make_stage_params(a.gamma_gas, a.Rgas, a.Cp, a.Prt, prm.c_rho, prm.c_ener);
     const int ugrid = std::min(RED_BLOCKS, std::max(1, grid_for(npoin_w, BLK)));
     for (int stage = 0; stage < 4; ++stage) {
@@ -988,6 +1034,10 @@ __global__ void k_loop_begin(int* __restrict__ ist) {
     if (threadIdx.x == 0) ist[0] += 1;
 }
 
+__global__ void k_copy_double(const double* __restrict__ src, double* __restrict__ dst) {
+    if (threadIdx.x == 0) *dst = *src;
+}
+
 // sc[0..n) = dt of each instance, sc[n..2n) = its time
 __global__ void k_loop_time(double* __restrict__ sc, int n) {
     for (int t = threadIdx.x; t < n; t += blockDim.x) sc[n + t] = sc[n + t] + sc[t];
@@ -1047,6 +1097,7 @@ __global__ __launch_bounds__(256) void k_ek_final_log(int npartial, const double
 void launch_adapt_dt(int nelem, int npoin, const int* connec, const double* helem, const double* u,
                      const double* csound, double cfl_conv, int has_diff, double cfl_diff,
                      const double* mu_fluid, const double* mu_sgs, const double* rho, double* d_dt_bits);
+void launch_hmin(int nelem, int npoin, const int* connec, const double* helem, double* hmin);
 void launch_adapt_dt_inst(int nelem, int npoin, const int* connec, const double* helem, const double* u,
                           const double* csound, double cfl_conv, int has_diff, double cfl_diff,
                           const double* mu_fluid, const double* mu_sgs, const double* rho, const int* elem_inst,
@@ -1073,7 +1124,8 @@ struct sod_time_loop {
     sod::FusedArgs a;       // level-2 views
     double *rho, *u, *q, *pr, *E, *Tem, *e_int, *eta;   // full level arrays
     double* Ngp = nullptr;  // device copy of the library's Ngp
-    double* sc = nullptr;   // dt[ninst], time[ninst]
+    double* sc = nullptr;   // dt[ninst], time[ninst], next dt[ninst] (folded CFL)
+    double* hmin = nullptr;  // per-node minimum element size (one mesh)
     int* ist = nullptr;     // istep, EK count
     int ninst = 1;          // > 1: a flattened ensemble described by B
     sod::BatchInfo B;
@@ -1121,7 +1173,13 @@ static void loop_enqueue_step(sod_time_loop* L, const LoopCfg& c) {
         add(L->Tem, 1, 2); add(L->e_int, 1, 2); add(L->eta, 1, 3);
         hipLaunchKernelGGL(k_copy_fields, dim3(cgrid.x, k), dim3(256), 0, s, (long)np, cl);
     }
-    enqueue_fused_step(L->P, a, L->sc, c.has_sgs, true, L->ninst > 1 ? &L->B : nullptr);
+    // the next step's CFL dt folded into the final update (one mesh, no sgs
+    // viscosity): its minimum collects in sc[2] and becomes dt after the
+    // time advance
+    const bool fold = !c.fixed_dt && L->ninst == 1 && !c.has_sgs && L->hmin != nullptr;
+    const DtFold f{L->hmin, L->sc + 2, c.cfl_conv, c.cfl_diff, c.has_diff};
+    if (fold) hipLaunchKernelGGL(k_set_double_f, dim3(1), dim3(64), 0, s, L->sc + 2, 100000000000000.0);
+    enqueue_fused_step(L->P, a, L->sc, c.has_sgs, true, L->ninst > 1 ? &L->B : nullptr, fold ? &f : nullptr);
     if (c.keep_levels) {
         // level 3 <- level 1 (CFDSolverBase.f90:1552-1557)
         CopyList cl{};
@@ -1144,7 +1202,9 @@ static void loop_enqueue_step(sod_time_loop* L, const LoopCfg& c) {
                            c.log_first, c.log_every, c.rho0, c.ek_cap, L->ek_hist);
     }
     if (!c.fixed_dt) {   // evalDt (CFDSolverBase.f90:1132-1141)
-        if (L->ninst > 1)
+        if (fold)
+            hipLaunchKernelGGL(k_copy_double, dim3(1), dim3(64), 0, s, L->sc + 2, L->sc);
+        else if (L->ninst > 1)
             launch_adapt_dt_inst(a.nelem, a.npoin, a.connec, a.helem, a.u, a.csound, c.cfl_conv, c.has_diff,
                                  c.cfl_diff, a.mu_fluid, a.mu_sgs, a.rho, L->B.elem_inst, L->ninst, L->sc);
         else
@@ -1234,7 +1294,14 @@ static int loop_create(sod_time_loop** out, int nelem, int npoin, int npoin_w, c
     int rc = prepare(L->P, nelem, npoin, npoin_w, dc, dl, dHe, dgv, dMl);
     if (rc == SOD_OK) {
         hipError_t e = hipMalloc(&L->Ngp, sizeof(double) * 64 * 64);
-        if (e == hipSuccess) e = hipMalloc(&L->sc, sizeof(double) * (2 * (size_t)ninst + 8));
+        if (e == hipSuccess) e = hipMalloc(&L->sc, sizeof(double) * (3 * (size_t)ninst + 8));
+        if (e == hipSuccess && ninst == 1) {
+            e = hipMalloc(&L->hmin, sizeof(double) * np);
+            if (e == hipSuccess) {
+                launch_hmin(nelem, npoin, dc, dh, L->hmin);
+                e = hipStreamSynchronize(rt().stream);
+            }
+        }
         if (e == hipSuccess) e = hipMalloc(&L->ist, sizeof(int) * 4);
         if (e == hipSuccess) e = hipMalloc(&L->ek_partial, sizeof(double) * RED_BLOCKS);
         if (e == hipSuccess)
@@ -1415,6 +1482,7 @@ extern "C" int sod_time_loop_destroy(sod_time_loop* L) {
     if (L->stream) (void)hipStreamDestroy(L->stream);
     (void)hipFree(L->Ngp);
     (void)hipFree(L->sc);
+    (void)hipFree(L->hmin);
     (void)hipFree(L->ist);
     (void)hipFree(L->ek_partial);
     (void)hipFree(L->ek_hist);

@@ -179,7 +179,9 @@ __global__ __launch_bounds__(256) void k_adapt_dt(int nelem, int npoin, const in
                                                   const double* __restrict__ mu_fluid,
                                                   const double* __restrict__ mu_sgs,
                                                   const double* __restrict__ rho,
+                                                  const int* __restrict__ run_flag,
                                                   unsigned long long* __restrict__ out_bits) {
+    if (run_flag != nullptr && *run_flag == 0) return;
     // A block takes tiles of 16 consecutive elements: thread (el, a0) reads
     // connec(e, a0 + 16r), r = 0..3, so each group of 16 lanes reads one
     // contiguous run of connec and every lane has four independent gathers in
@@ -233,6 +235,105 @@ __global__ __launch_bounds__(256) void k_adapt_dt(int nelem, int npoin, const in
         if (threadIdx.x == 0 && best < HUGE_VAL)
             atomicMin(out_bits, (unsigned long long)__double_as_longlong(best));
     }
+}
+
+// Per-node minimum element size h_min(a) = min over the elements e holding
+// node a of helem(e), as the bits of a positive double (ordered like the
+// value); nodes no element holds keep +inf.  Skipped when `run_flag` points
+// at 0.
+constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
+
+__global__ __launch_bounds__(256) void k_fill_u64(long n, unsigned long long* __restrict__ p,
+                                                  unsigned long long v, const int* __restrict__ run_flag) {
+    if (run_flag != nullptr && *run_flag == 0) return;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_hmin_build(int nelem, const int* __restrict__ connec,
+                                                    const double* __restrict__ helem,
+                                                    unsigned long long* __restrict__ hmin_bits,
+                                                    const int* __restrict__ run_flag) {
+    if (run_flag != nullptr && *run_flag == 0) return;
+    const long n = (long)nelem * 64;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const int e = (int)(t % nelem);   // connec(e, a) at e + nelem*a: t runs through it in memory order
+        atomicMin(hmin_bits + (connec[t] - 1), (unsigned long long)__double_as_longlong(helem[e]));
+    }
+}
+
+// adapt_dt_cfl regrouped by node (node_dt): one pass over the nodes with
+// their cached h_min, no connectivity gather.  Skipped when `skip_flag`
+// points at a non-zero value (mu_sgs non-zero: the element form runs).
+__global__ __launch_bounds__(256) void k_adapt_dt_nodes(int npoin, const double* __restrict__ hmin,
+                                                        const double* __restrict__ u,
+                                                        const double* __restrict__ csound, double cfl_conv,
+                                                        int has_diff, double cfl_diff,
+                                                        const double* __restrict__ mu_fluid,
+                                                        const double* __restrict__ rho,
+                                                        const int* __restrict__ skip_flag,
+                                                        unsigned long long* __restrict__ out_bits) {
+    __shared__ double s_m[4];
+    if (skip_flag != nullptr && *skip_flag != 0) return;
+    const size_t np = (size_t)npoin;
+    double best = HUGE_VAL;
+    for (int n = blockIdx.x * 256 + threadIdx.x; n < npoin; n += gridDim.x * 256) {
+        const double h = hmin[n];
+        if (!(h < HUGE_VAL)) continue;   // a node no element holds
+        best = fmin(best, node_dt(h, u[n], u[n + np], u[n + 2 * np], csound[n], has_diff ? mu_fluid[n] : 0.0,
+                                  has_diff ? rho[n] : 1.0, cfl_conv, has_diff, cfl_diff));
+    }
+    best = wave_min(best);
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double v = fmin(fmin(s_m[0], s_m[1]), fmin(s_m[2], s_m[3]));
+        if (v < HUGE_VAL) atomicMin(out_bits, (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+// Whether the cached h_min of the drop-in CFL entry point still belongs to
+// (connec, helem): a position-weighted fingerprint of 4096 sampled entries
+// of each is compared with the stored one; *stale = 1 when they differ (or
+// `force`), and the new fingerprint is stored.  One block of 256 threads.
+__global__ __launch_bounds__(256) void k_dt_cache_check(int nelem, const int* __restrict__ connec,
+                                                        const double* __restrict__ helem,
+                                                        double* __restrict__ fp, int* __restrict__ stale,
+                                                        int force) {
+    __shared__ double s_a[256], s_b[256];
+    const int t = threadIdx.x;
+    const long n = (long)nelem * 64;
+    double a = 0.0, b = 0.0;
+    for (int smp = 0; smp < 16; ++smp) {
+        const long k = t + 256L * smp;
+        const double w = 1.0 + 0.001 * (double)k;
+        a += w * (double)connec[(n * k) / 4096];
+        b += w * helem[((long)nelem * k) / 4096];
+    }
+    s_a[t] = a;
+    s_b[t] = b;
+    __syncthreads();
+    if (t == 0) {
+        double sa = 0.0, sb = 0.0;
+        for (int q = 0; q < 256; ++q) {
+            sa += s_a[q];
+            sb += s_b[q];
+        }
+        const bool same = !force && fp[0] == sa && fp[1] == sb;
+        *stale = same ? 0 : 1;
+        fp[0] = sa;
+        fp[1] = sb;
+    }
+}
+
+// flag = 1 if any of the n values is non-zero
+__global__ __launch_bounds__(256) void k_any_nonzero(long n, const double* __restrict__ v, int* __restrict__ flag) {
+    int nz = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) nz |= v[i] != 0.0;
+    if (__any(nz) && (threadIdx.x & 63) == 0) *flag = 1;
+}
+
+__global__ void k_zero_ints(int n, int* __restrict__ p) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) p[t] = 0;
 }
 
 // adapt_dt_cfl of each instance of a flattened ensemble: element e belongs to
@@ -533,8 +634,85 @@ void launch_adapt_dt(int nelem, int npoin, const int* connec, const double* hele
     hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, s, d_dt_bits, init);
     if (nelem > 0)
         hipLaunchKernelGGL(k_adapt_dt, dim3(std::min(2048, grid_for(nelem, 16))), dim3(256), 0, s, nelem, npoin, connec,
-                           helem, u, csound, cfl_conv, has_diff, cfl_diff, mu_fluid, mu_sgs, rho,
+                           helem, u, csound, cfl_conv, has_diff, cfl_diff, mu_fluid, mu_sgs, rho, nullptr,
                            reinterpret_cast<unsigned long long*>(d_dt_bits));
+}
+
+void launch_hmin(int nelem, int npoin, const int* connec, const double* helem, double* hmin) {
+    hipStream_t s = rt().stream;
+    auto* bits = reinterpret_cast<unsigned long long*>(hmin);
+    if (npoin > 0)
+        hipLaunchKernelGGL(k_fill_u64, dim3(std::min(1024, grid_for(npoin, 256))), dim3(256), 0, s, (long)npoin,
+                           bits, kInfBits, nullptr);
+    if (nelem > 0)
+        hipLaunchKernelGGL(k_hmin_build, dim3(std::min(2048, grid_for((long)nelem * 64, 256))), dim3(256), 0, s,
+                           nelem, connec, helem, bits, nullptr);
+}
+
+// Cached per-node h_min of the drop-in CFL entry point, keyed on the device
+// addresses of connec and helem and checked on the device against a
+// fingerprint of their contents on every call (a stale cache is rebuilt by
+// the same launches, without a host synchronisation).
+struct DtNodeCache {
+    const int* connec = nullptr;
+    const double* helem = nullptr;
+    int nelem = -1, npoin = -1;
+    double* hmin = nullptr;
+    double* fp = nullptr;   // stored fingerprint (2)
+    int* flags = nullptr;   // [0] cache stale, [1] mu_sgs holds a non-zero value
+    void release() {
+        (void)hipFree(hmin);
+        (void)hipFree(fp);
+        (void)hipFree(flags);
+        *this = DtNodeCache();
+    }
+};
+
+static DtNodeCache g_dtc;
+
+void dt_cache_release() { g_dtc.release(); }
+
+// adapt_dt_cfl through the node form when mu_sgs is absent or zero (decided
+// on the device), through the element form otherwise; result bits in
+// d_dt_bits.
+int launch_adapt_dt_cached(int nelem, int npoin, const int* connec, const double* helem, const double* u,
+                           const double* csound, double cfl_conv, int has_diff, double cfl_diff,
+                           const double* mu_fluid, const double* mu_sgs, const double* rho, double* d_dt_bits) {
+    hipStream_t s = rt().stream;
+    DtNodeCache& C = g_dtc;
+    int force = 0;
+    if (C.connec != connec || C.helem != helem || C.nelem != nelem || C.npoin != npoin) {
+        SOD_HIP(hipStreamSynchronize(s));
+        C.release();
+        SOD_HIP(hipMalloc(&C.hmin, sizeof(double) * (size_t)std::max(1, npoin)));
+        SOD_HIP(hipMalloc(&C.fp, sizeof(double) * 2));
+        SOD_HIP(hipMalloc(&C.flags, sizeof(int) * 2));
+        C.connec = connec;
+        C.helem = helem;
+        C.nelem = nelem;
+        C.npoin = npoin;
+        force = 1;
+    }
+    auto* out = reinterpret_cast<unsigned long long*>(d_dt_bits);
+    auto* hbits = reinterpret_cast<unsigned long long*>(C.hmin);
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, s, d_dt_bits, 100000000000000.0);
+    if (nelem <= 0 || npoin <= 0) return SOD_OK;
+    hipLaunchKernelGGL(k_dt_cache_check, dim3(1), dim3(256), 0, s, nelem, connec, helem, C.fp, C.flags, force);
+    hipLaunchKernelGGL(k_fill_u64, dim3(std::min(1024, grid_for(npoin, 256))), dim3(256), 0, s, (long)npoin, hbits,
+                       kInfBits, C.flags);
+    hipLaunchKernelGGL(k_hmin_build, dim3(std::min(2048, grid_for((long)nelem * 64, 256))), dim3(256), 0, s, nelem,
+                       connec, helem, hbits, C.flags);
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, 1, C.flags + 1);
+    if (has_diff)
+        hipLaunchKernelGGL(k_any_nonzero, dim3(std::min(1024, grid_for((long)nelem * 64, 256))), dim3(256), 0, s,
+                           (long)nelem * 64, mu_sgs, C.flags + 1);
+    hipLaunchKernelGGL(k_adapt_dt_nodes, dim3(std::min(1024, grid_for(npoin, 256))), dim3(256), 0, s, npoin,
+                       C.hmin, u, csound, cfl_conv, has_diff, cfl_diff, mu_fluid, rho, C.flags + 1, out);
+    if (has_diff)
+        hipLaunchKernelGGL(k_adapt_dt, dim3(std::min(2048, grid_for(nelem, 16))), dim3(256), 0, s, nelem, npoin,
+                           connec, helem, u, csound, cfl_conv, has_diff, cfl_diff, mu_fluid, mu_sgs, rho,
+                           C.flags + 1, out);
+    return SOD_OK;
 }
 
 void launch_adapt_dt_inst(int nelem, int npoin, const int* connec, const double* helem, const double* u,
@@ -650,8 +828,8 @@ int sod_adapt_dt_cfl(int nelem, int npoin, const int* connec, const double* hele
     }
     void* work = nullptr;
     SOD_TRY(scratch(64, &work, 1));
-    launch_adapt_dt(nelem, npoin, dc, dh, du, dcs, cfl_conv, has_diff, has_diff ? *cfl_diff : 0.0, dmu,
-                    dsgs, drho, static_cast<double*>(work));
+    SOD_TRY(launch_adapt_dt_cached(nelem, npoin, dc, dh, du, dcs, cfl_conv, has_diff, has_diff ? *cfl_diff : 0.0,
+                                   dmu, dsgs, drho, static_cast<double*>(work)));
     SOD_HIP(hipGetLastError());
     SOD_HIP(hipStreamSynchronize(rt().stream));
     SOD_HIP(hipMemcpy(dt, work, sizeof(double), hipMemcpyDeviceToHost));

@@ -14,7 +14,8 @@
 namespace sod {
 
 int upload_tables();   // tables.hip
-void fused_release();  // k_fused.hip
+void fused_release();     // k_fused.hip
+void dt_cache_release();  // k_misc.hip
 
 namespace {
 thread_local char g_err[2048] = "";
@@ -210,6 +211,7 @@ int sod_finalize(void) {
     g_scratch.clear();
     g_tables_ok.clear();
     fused_release();
+    dt_cache_release();
     rt().initialised = false;
     return SOD_OK;
 }

@@ -33,6 +33,29 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + min(x, r8) + (b >> 3);
 }
 
+// CFL time step candidate of one node (adapt_dt_cfl, mod_time_ops.f90:37-52,
+// with mu_sgs = 0).  The reference forms, per element e,
+//   cfl_c*(h_e/7)/max_a(|u_a|_inf + c_a)  and  cfl_d*(h_e/7)^2/max_a(mu_a/rho_a)
+// and takes the minimum over elements.  Every rounded operation there is
+// monotone (non-decreasing in h_e, non-increasing in the maxima), so that
+// minimum equals, bit for bit, the minimum over working nodes a of the same
+// expressions evaluated with h = min over the elements holding a of h_e and
+// the node's own values.  The arithmetic below is the per-element one.
+__device__ __forceinline__ double node_dt(double hmin, double u0, double u1, double u2, double c, double mu,
+                                          double r, double cfl_c, int has_diff, double cfl_d) {
+    double umag = fabs(u0);
+    umag = fmax(umag, fabs(u1));
+    umag = fmax(umag, fabs(u2));
+    const double L3 = fmax(0.0, umag + c);
+    const double hh = hmin / 7.0;
+    double dt = cfl_c * hh / L3;
+    if (has_diff) {
+        const double mx = fmax(0.0, (r * 0.0 + mu) / r);
+        dt = fmin(dt, cfl_d * (hh * hh) / mx);
+    }
+    return dt;
+}
+
 // orders LDS accesses of the lanes of one wavefront
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -208,6 +208,43 @@ def test_smart_visc_dt_ek(case):
     assert abs(ek.value / ek_ref - 1) < 1e-13
 
 
+def test_adapt_dt_node_form_cache_and_sgs(case):
+    """sod_adapt_dt_cfl reduces over nodes with a cached per-node minimum
+    element size when mu_sgs is zero, and over elements otherwise: both
+    equal the oracle bit for bit, and rewriting helem in place (same device
+    address) rebuilds the cache."""
+    name, m, g, gas, s = case
+    ne, npn = m.nelem, m.npoin
+    D = Dev()
+    c, hp = D("c", m.connec), D("h", g["helem"])
+    args = dict(u=D("u", s["u"]), cs=D("cs", s["csound"]), mu=D("mu", s["mu_fluid"]), r=D("r", s["rho"]))
+    rng = np.random.default_rng(13)
+    sgs_nz = np.asfortranarray(rng.uniform(0, 2e-3, (ne, 64)))
+    zero = np.zeros((ne, 64), order="F")
+    sg0, sg1 = D("sg0", zero), D("sg1", sgs_nz)
+
+    def run(sgp, cfl_diff=True):
+        dt = ctypes.c_double(0.0)
+        cd = ctypes.c_double(0.5)
+        _lib.call("sod_adapt_dt_cfl", ne, npn, c, hp, args["u"], args["cs"], 0.45, ctypes.addressof(dt),
+                  ctypes.addressof(cd) if cfl_diff else None, args["mu"], sgp, args["r"])
+        return dt.value
+
+    def ref(helem, sgs, cfl_diff=True):
+        if not cfl_diff:
+            return oracle.adapt_dt_cfl(m.connec, helem, s["u"], s["csound"], 0.45, 1e300, s["mu_fluid"],
+                                       sgs, s["rho"])
+        return oracle.adapt_dt_cfl(m.connec, helem, s["u"], s["csound"], 0.45, 0.5, s["mu_fluid"], sgs, s["rho"])
+
+    for rep in range(2):   # second call reuses the cache
+        assert run(sg0) == ref(g["helem"], zero), rep
+    assert run(sg1) == ref(g["helem"], sgs_nz)
+    assert run(sg0, cfl_diff=False) == ref(g["helem"], zero, cfl_diff=False)
+    h2 = np.asfortranarray(g["helem"] * rng.uniform(0.5, 1.5, ne))
+    D.t["h"].copy_(torch.from_numpy(np.ascontiguousarray(h2)).to("cuda"))
+    assert run(sg0) == ref(h2, zero)
+
+
 def _oracle_steps(m, g, gas, s, dts):
     s = {k: v.copy(order="F") for k, v in s.items()}
     p = oracle.default_params(Rgas_inf=gas.Rgas)
