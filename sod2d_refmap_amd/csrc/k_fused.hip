@@ -45,11 +45,11 @@ constexpr int BLK = 256;
 // the two the unconstrained register allocation gives, despite a few spills.
 template <bool HAS_SGS>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_stage_rhs(
-    int nelem, int npoin, const int* __restrict__ conn, const double* __restrict__ He,
+    int nelem, const int* __restrict__ conn, const double* __restrict__ He,
     const double* __restrict__ gpv, const double* __restrict__ mue, const double* __restrict__ musgs,
     const double* __restrict__ rho, const double* __restrict__ q0, const double* __restrict__ q1,
     const double* __restrict__ q2, const double* __restrict__ Ein, const double* __restrict__ mufac,
-    double* __restrict__ R, StageParams P) {
+    double* __restrict__ relem, StageParams P) {
     __shared__ double sh[WPB][18][64];
     // The derivative matrix is copied to LDS with a flat index first:
     // indexing the __constant__ 2-D table with a lane-dependent row gave
@@ -73,7 +73,6 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))
     if (e >= nelem) return;   // uniform per wavefront; only wave-level sync below
     const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
     const size_t eb = (size_t)e * 64;
-    const size_t np = (size_t)npoin;
 
     // ---- gather + equation of state (time_integ.f90:266-285) -----------
     const int n = conn[eb + l];
@@ -261,13 +260,52 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))
     const double nu_e = P.c_rho * mu_e * ri;
     const double kappa = muf * P.CpPr + P.c_ener * mu_e * P.CpPr + P.Cp09 * r * mu_s;
 
-    double* Rn = R + n;
-    unsafeAtomicAdd(Rn + R_MASS * np, gv * Rm + nu_e * dD[0]);
-    unsafeAtomicAdd(Rn + R_MOM0 * np, gv * Rq[0] + dD[1]);
-    unsafeAtomicAdd(Rn + R_MOM1 * np, gv * Rq[1] + dD[2]);
-    unsafeAtomicAdd(Rn + R_MOM2 * np, gv * Rq[2] + dD[3]);
-    unsafeAtomicAdd(Rn + R_ENER * np, gv * Re + dD[4] + kappa * dD[5]);
-    unsafeAtomicAdd(Rn + R_ETA * np, gv * Reta);
+    // The residuals of this element at this node, stored in element-node
+    // order; k_stage_update and k_assemble_nodes sum each node's
+    // contributions (pull assembly: no atomics).
+    double2* o = reinterpret_cast<double2*>(relem + (eb + l) * NRES);
+    o[0] = make_double2(gv * Rm + nu_e * dD[0], gv * Rq[0] + dD[1]);
+    o[1] = make_double2(gv * Rq[1] + dD[2], gv * Rq[2] + dD[3]);
+    o[2] = make_double2(gv * Re + dD[4] + kappa * dD[5], gv * Reta);
+}
+
+// Sum of the element-node residuals that belong to node n: the positions
+// p = e*64 + lane of its occurrences in the lane-layout connectivity,
+// ascending (a fixed order, so the assembled residual is reproducible).
+__device__ __forceinline__ void node_residuals(int n, const int* __restrict__ inv_off, const int* __restrict__ inv_pos,
+                                               const double* __restrict__ relem, double acc[NRES]) {
+#pragma unroll
+    for (int f = 0; f < NRES; ++f) acc[f] = 0.0;
+    const int q1 = inv_off[n + 1];
+    for (int q = inv_off[n]; q < q1; ++q) {
+        const double2* c = reinterpret_cast<const double2*>(relem + (size_t)inv_pos[q] * NRES);
+        const double2 a = c[0], b = c[1], d = c[2];
+        acc[0] += a.x;
+        acc[1] += a.y;
+        acc[2] += b.x;
+        acc[3] += b.y;
+        acc[4] += d.x;
+        acc[5] += d.y;
+    }
+}
+
+// Assembled residuals of every node into the caller's arrays (sod_full_rhs).
+__global__ __launch_bounds__(256) void k_assemble_nodes(int npoin, const int* __restrict__ inv_off,
+                                                        const int* __restrict__ inv_pos,
+                                                        const double* __restrict__ relem, double* __restrict__ Rmass,
+                                                        double* __restrict__ Rmom, double* __restrict__ Rener,
+                                                        double* __restrict__ Reta) {
+    const size_t np = (size_t)npoin;
+    for (int n = blockIdx.x * 256 + threadIdx.x; n < npoin; n += gridDim.x * 256) {
+        double acc[NRES];
+        node_residuals(n, inv_off, inv_pos, relem, acc);
+        Rmass[n] = acc[R_MASS];
+        Rmom[n] = acc[R_MOM0];
+        Rmom[n + np] = acc[R_MOM1];
+        Rmom[n + 2 * np] = acc[R_MOM2];
+        Rener[n] = acc[R_ENER];
+        Reta[n] = acc[R_ETA];
+    }
 }
 
 // ------------------------------------------------------------ node kernels
@@ -281,8 +319,11 @@ struct StepState {
     const double *Ml, *mu_factor;
     // library arrays
     double *aux;    // 5 x npoin: rho, q0, q1, q2, E of the next stage
-    double *R;      // 6 x npoin
     double *Rsum;   // 6 x npoin
+    // element-node residuals of the stage kernel and the node -> (element,
+    // lane) positions they are summed over
+    const double* relem;
+    const int *inv_off, *inv_pos;
     double *partial;   // reduction partials (RED_BLOCKS)
     // CFL time step of the next step folded into the final update (node_dt;
     // one mesh, mu_sgs = 0): per-node h_min, and the bits of the running
@@ -315,11 +356,9 @@ __global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, co
         const double dt = BATCHED ? d_dt[node_inst[n]] : d_dt[0];
         const double ml = st.Ml[n];
         double Rf[NRES];
+        node_residuals(n, st.inv_off, st.inv_pos, st.relem, Rf);
 #pragma unroll
-        for (int f = 0; f < NRES; ++f) {
-            Rf[f] = st.R[f * np + n] / ml;   // mod_solver.f90:32-36
-            st.R[f * np + n] = 0.0;          // ready for the next stage
-        }
+        for (int f = 0; f < NRES; ++f) Rf[f] = Rf[f] / ml;   // mod_solver.f90:32-36
         const double b = k_b[s];
         double Rs[NRES];
 #pragma unroll
@@ -670,7 +709,10 @@ struct Prepared {
     double* musgs = nullptr;
     int* slaves = nullptr;
     int nslave = 0;
-    double* work = nullptr;   // aux(5) + R(6) + Rsum(6) per npoin, + partials
+    double* work = nullptr;   // aux(5) + Rsum(6) per npoin, + partials
+    double* relem = nullptr;  // nelem*64*6 element-node residuals of the stage kernel
+    int* inv_off = nullptr;   // npoin+1: node n's positions are inv_pos[inv_off[n] .. inv_off[n+1])
+    int* inv_pos = nullptr;   // nelem*64 positions e*64 + lane, ascending per node
     int* flag = nullptr;
     double* dscal = nullptr;  // device scalars: [0] dt of the step
     double* fpbuf = nullptr;
@@ -684,6 +726,9 @@ struct Prepared {
         (void)hipFree(musgs);
         (void)hipFree(slaves);
         (void)hipFree(work);
+        (void)hipFree(relem);
+        (void)hipFree(inv_off);
+        (void)hipFree(inv_pos);
         (void)hipFree(flag);
         (void)hipFree(dscal);
         (void)hipFree(fpbuf);
@@ -691,9 +736,32 @@ struct Prepared {
     }
 };
 
-static Prepared g_prep;
+// Prepared meshes of the drop-in entry points.  A few recent meshes are kept
+// (up to kPrepSlots of them and kPrepBytes of device memory), so that a
+// caller alternating between meshes -- the members of an ensemble stepped one
+// after the other -- does not rebuild them on every call.
+constexpr int kPrepSlots = 64;
+constexpr size_t kPrepBytes = size_t(32) << 30;
+struct PrepSlot {
+    Prepared P;
+    unsigned long long used = 0;
+    size_t bytes = 0;
+};
+static PrepSlot g_slots[kPrepSlots];
+static unsigned long long g_prep_clock = 0;
 
-void fused_release() { g_prep.release(); }
+static size_t prepared_bytes(int nelem, int npoin) {
+    // conn, He, gpvol, mu_e, mu_sgs, element-node residuals and positions per
+    // element-node; work, inverse offsets per node
+    return (size_t)nelem * 64 * (4 + 72 + 8 + 8 + 8 + 48 + 4) + (size_t)npoin * (11 * 8 + 4);
+}
+
+void fused_release() {
+    for (auto& sl : g_slots) {
+        sl.P.release();
+        sl = PrepSlot();
+    }
+}
 
 // connectivity ids must lie in 1..npoin (the kernels gather and scatter
 // through them without further checks)
@@ -731,7 +799,7 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
     SOD_HIP(hipMalloc(&P.flag, sizeof(int) * 4));
     SOD_HIP(hipMalloc(&P.dscal, sizeof(double) * 8));
     SOD_HIP(hipMalloc(&P.fpbuf, sizeof(double) * 4));
-    P.work_doubles = np * 17 + 2 * RED_BLOCKS + 64;
+    P.work_doubles = np * 11 + 2 * RED_BLOCKS + 64;
     SOD_HIP(hipMalloc(&P.work, sizeof(double) * P.work_doubles));
     SOD_HIP(hipMemsetAsync(P.work, 0, sizeof(double) * P.work_doubles, rt().stream));
     SOD_HIP(hipMemsetAsync(P.musgs, 0, sizeof(double) * ne * 64, rt().stream));
@@ -752,6 +820,22 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
         return fail(SOD_ERR_ARG, "connec holds an id outside 1..npoin");
     }
     SOD_HIP(hipMemcpy(hw.data(), lpoin_w, sizeof(int) * npoin_w, hipMemcpyDeviceToHost));
+    {
+        // node -> (element, lane) positions of the lane-layout connectivity,
+        // by a counting sort that keeps the positions ascending
+        const size_t nl = ne * 64;
+        std::vector<int> cl(nl), off(np + 1, 0), pos(nl);
+        if (nl > 0) SOD_HIP(hipMemcpy(cl.data(), P.conn, sizeof(int) * nl, hipMemcpyDeviceToHost));
+        for (size_t t = 0; t < nl; ++t) ++off[(size_t)cl[t] + 1];
+        for (size_t n = 0; n < np; ++n) off[n + 1] += off[n];
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (size_t t = 0; t < nl; ++t) pos[fill[cl[t]]++] = (int)t;
+        SOD_HIP(hipMalloc(&P.inv_off, sizeof(int) * (np + 1)));
+        SOD_HIP(hipMalloc(&P.inv_pos, sizeof(int) * (nl + 1)));
+        SOD_HIP(hipMalloc(&P.relem, sizeof(double) * (nl * NRES + 2)));
+        SOD_HIP(hipMemcpy(P.inv_off, off.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice));
+        if (nl > 0) SOD_HIP(hipMemcpy(P.inv_pos, pos.data(), sizeof(int) * nl, hipMemcpyHostToDevice));
+    }
     std::vector<char> isw(npoin, 0);
     for (int t = 0; t < npoin_w; ++t) {
         if (hw[t] < 1 || hw[t] > npoin) {
@@ -777,6 +861,55 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
     P.nelem = nelem;
     P.npoin = npoin;
     P.npoin_w = npoin_w;
+    return SOD_OK;
+}
+
+// The prepared mesh of these arguments from the drop-in cache (built or
+// rebuilt by prepare() as needed).
+static int prepare_cached(Prepared** out, int nelem, int npoin, int npoin_w, const int* connec,
+                          const int* lpoin_w, const double* He, const double* gpvol, const double* Ml) {
+    const void* key[5] = {connec, lpoin_w, He, gpvol, Ml};
+    PrepSlot* hit = nullptr;
+    for (auto& sl : g_slots) {
+        bool same = sl.used != 0 && sl.P.nelem == nelem && sl.P.npoin == npoin && sl.P.npoin_w == npoin_w;
+        for (int t = 0; t < 5 && same; ++t) same = sl.P.key[t] == key[t];
+        if (same) {
+            hit = &sl;
+            break;
+        }
+    }
+    const size_t need = prepared_bytes(nelem, npoin);
+    if (hit == nullptr) {
+        // evict least recently used meshes until the new one fits
+        while (true) {
+            size_t total = 0;
+            PrepSlot* lru = nullptr;
+            PrepSlot* empty = nullptr;
+            for (auto& sl : g_slots) {
+                if (sl.used == 0) {
+                    if (!empty) empty = &sl;
+                    continue;
+                }
+                total += sl.bytes;
+                if (!lru || sl.used < lru->used) lru = &sl;
+            }
+            if (empty && (total + need <= kPrepBytes || lru == nullptr)) {
+                hit = empty;
+                break;
+            }
+            SOD_HIP(hipStreamSynchronize(rt().stream));   // queued work may still use it
+            lru->P.release();
+            *lru = PrepSlot();
+        }
+    }
+    hit->used = ++g_prep_clock;
+    hit->bytes = need;
+    const int rc = prepare(hit->P, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml);
+    if (rc != SOD_OK) {
+        *hit = PrepSlot();
+        return rc;
+    }
+    *out = &hit->P;
     return SOD_OK;
 }
 
@@ -858,9 +991,11 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
     st.Ml = a.Ml;
     st.mu_factor = a.mu_factor;
     st.aux = P.work;
-    st.R = P.work + 5 * np;
-    st.Rsum = P.work + 11 * np;
-    st.partial = P.work + 17 * np;
+    st.Rsum = P.work + 5 * np;
+    st.partial = P.work + 11 * np;
+    st.relem = P.relem;
+    st.inv_off = P.inv_off;
+    st.inv_pos = P.inv_pos;
     st.hmin = nullptr;
     st.dt_bits = nullptr;
     st.cfl_conv = st.cfl_diff = 0.0;
@@ -885,12 +1020,12 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
             SOD_PROF("stage_rhs");
             if (has_sgs)
                 hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem,
-                                   npoin, P.conn, P.He, P.gpv, P.mue, P.musgs, ir, i0, i1, i2, iE, a.mu_factor,
-                                   st.R, sp);
+                                   P.conn, P.He, P.gpv, P.mue, P.musgs, ir, i0, i1, i2, iE, a.mu_factor, P.relem,
+                                   sp);
             else
                 hipLaunchKernelGGL(k_stage_rhs<false>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem,
-                                   npoin, P.conn, P.He, P.gpv, P.mue, P.musgs, ir, i0, i1, i2, iE, a.mu_factor,
-                                   st.R, sp);
+                                   P.conn, P.He, P.gpv, P.mue, P.musgs, ir, i0, i1, i2, iE, a.mu_factor, P.relem,
+                                   sp);
         }
         SOD_PROF("stage_update");
         const int* ni = B ? B->node_inst : nullptr;
@@ -945,8 +1080,9 @@ int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const i
                    const double* mu_factor, double* rho, double* u, double* q, double* pr, double* E,
                    double* Tem, double* e_int, double* eta, double* csound, double* machno,
                    double* mu_fluid, double* mu_e, const double* mu_sgs, double* mue_l) {
-    Prepared& P = g_prep;
-    SOD_TRY(prepare(P, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
+    Prepared* Pp = nullptr;
+    SOD_TRY(prepare_cached(&Pp, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
+    Prepared& P = *Pp;
     int has_sgs = 0;
     SOD_TRY(load_viscosities(P, nelem, mu_e, mu_sgs, &has_sgs));
     const FusedArgs a{nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml, helem, helem_l, mu_factor, Rgas,
@@ -961,8 +1097,9 @@ int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const i
 int prepare_lane_mesh(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
                       const double* He, const double* gpvol, const double* Ml, const double* mu_e,
                       const double* mu_sgs, LaneMesh* out) {
-    Prepared& P = g_prep;
-    SOD_TRY(prepare(P, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
+    Prepared* Pp = nullptr;
+    SOD_TRY(prepare_cached(&Pp, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
+    Prepared& P = *Pp;
     int has_sgs = 0;
     SOD_TRY(load_viscosities(P, nelem, mu_e, mu_sgs, &has_sgs));
     *out = LaneMesh{P.conn, P.He, P.gpv, P.mue, P.musgs, has_sgs != 0};
@@ -977,33 +1114,29 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
              const double* gpvol, const double* Ml, const double* rho, const double* q, const double* E,
              const double* mu_factor, const double* mu_e, const double* mu_sgs, double Rgas, double gamma_gas,
              double Cp, double Prt, double* Rmass, double* Rmom, double* Rener, double* Reta) {
-    Prepared& Pm = g_prep;
-    SOD_TRY(prepare(Pm, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
+    Prepared* Pp = nullptr;
+    SOD_TRY(prepare_cached(&Pp, nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, Ml));
+    Prepared& Pm = *Pp;
     hipStream_t s = rt().stream;
     const sod_numerical_params& prm = rt().params;
     const size_t np = (size_t)npoin;
-    void* buf = nullptr;
-    SOD_TRY(scratch(sizeof(double) * np * NRES, &buf, 5));
-    double* R = static_cast<double*>(buf);
-    SOD_HIP(hipMemsetAsync(R, 0, sizeof(double) * np * NRES, s));
     int has_sgs = 0;
     SOD_TRY(load_viscosities(Pm, nelem, mu_e, mu_sgs, &has_sgs));
     if (nelem > 0) {
         const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
         SOD_PROF("full_rhs");
         if (has_sgs)
-            hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, npoin,
-                               Pm.conn, Pm.He, Pm.gpv, Pm.mue, Pm.musgs, rho, q, q + np, q + 2 * np, E, mu_factor,
-                               R, sp);
+            hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, Pm.conn,
+                               Pm.He, Pm.gpv, Pm.mue, Pm.musgs, rho, q, q + np, q + 2 * np, E, mu_factor, Pm.relem,
+                               sp);
         else
-            hipLaunchKernelGGL(k_stage_rhs<false>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, npoin,
-                               Pm.conn, Pm.He, Pm.gpv, Pm.mue, Pm.musgs, rho, q, q + np, q + 2 * np, E, mu_factor,
-                               R, sp);
+            hipLaunchKernelGGL(k_stage_rhs<false>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, Pm.conn,
+                               Pm.He, Pm.gpv, Pm.mue, Pm.musgs, rho, q, q + np, q + 2 * np, E, mu_factor, Pm.relem,
+                               sp);
     }
-    SOD_HIP(hipMemcpyAsync(Rmass, R + R_MASS * np, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
-    SOD_HIP(hipMemcpyAsync(Rmom, R + R_MOM0 * np, sizeof(double) * np * 3, hipMemcpyDeviceToDevice, s));
-    SOD_HIP(hipMemcpyAsync(Rener, R + R_ENER * np, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
-    SOD_HIP(hipMemcpyAsync(Reta, R + R_ETA * np, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+    if (npoin > 0)
+        hipLaunchKernelGGL(k_assemble_nodes, dim3(std::min(2048, grid_for(npoin, 256))), dim3(256), 0, s, npoin,
+                           Pm.inv_off, Pm.inv_pos, Pm.relem, Rmass, Rmom, Rener, Reta);
     SOD_HIP(hipGetLastError());
     return SOD_OK;
 }
