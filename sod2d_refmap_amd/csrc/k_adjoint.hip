@@ -47,12 +47,12 @@ enum {
 };
 
 template <bool HAS_SGS>
-__global__ __launch_bounds__(64 * AWPB) void k_stage_rhs_vjp(
+__global__ __launch_bounds__(64 * AWPB) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_stage_rhs_vjp(
     int nelem, int npoin, const int* __restrict__ conn, const double* __restrict__ He,
     const double* __restrict__ gpv, const double* __restrict__ mue, const double* __restrict__ musgs,
     const double* __restrict__ rho, const double* __restrict__ q0, const double* __restrict__ q1,
     const double* __restrict__ q2, const double* __restrict__ Ein, const double* __restrict__ mufac,
-    const double* __restrict__ lam, double* __restrict__ grad, double* __restrict__ gmue_lane, StageParams P) {
+    const double* __restrict__ lam, double* __restrict__ relem, double* __restrict__ gmue_lane, StageParams P) {
     __shared__ double sh[AWPB][NROW][64];
     __shared__ double sD[16];
     if (threadIdx.x < 16) sD[threadIdx.x] = (&c_tab.D[0][0])[threadIdx.x];
@@ -444,12 +444,39 @@ __global__ __launch_bounds__(64 * AWPB) void k_stage_rhs_vjp(
             Lri += Lu * mm[a];
         }
         Lr += -ri * ri * Lri;
-        unsafeAtomicAdd(grad + n, Lr);
-        unsafeAtomicAdd(grad + np + n, Lm[0]);
-        unsafeAtomicAdd(grad + 2 * np + n, Lm[1]);
-        unsafeAtomicAdd(grad + 3 * np + n, Lm[2]);
-        unsafeAtomicAdd(grad + 4 * np + n, Len);
+        // gradients at this element node; k_assemble_grad sums them per node
+        double2* o = reinterpret_cast<double2*>(relem + (eb + l) * 6);
+        o[0] = make_double2(Lr, Lm[0]);
+        o[1] = make_double2(Lm[1], Lm[2]);
+        o[2] = make_double2(Len, 0.0);
         gmue_lane[eb + l] = L_mue;
+    }
+}
+
+// Per-node sums of the element-node gradients of k_stage_rhs_vjp (the
+// positions of node n, ascending) into the caller's arrays.
+__global__ __launch_bounds__(256) void k_assemble_grad(int npoin, const int* __restrict__ inv_off,
+                                                       const int* __restrict__ inv_pos,
+                                                       const double* __restrict__ relem, double* __restrict__ g_rho,
+                                                       double* __restrict__ g_q, double* __restrict__ g_E) {
+    const size_t np = (size_t)npoin;
+    for (int n = blockIdx.x * 256 + threadIdx.x; n < npoin; n += gridDim.x * 256) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+        const int q1 = inv_off[n + 1];
+        for (int q = inv_off[n]; q < q1; ++q) {
+            const double2* c = reinterpret_cast<const double2*>(relem + (size_t)inv_pos[q] * 6);
+            const double2 x = c[0], y = c[1], z = c[2];
+            a0 += x.x;
+            a1 += x.y;
+            a2 += y.x;
+            a3 += y.y;
+            a4 += z.x;
+        }
+        g_rho[n] = a0;
+        g_q[n] = a1;
+        g_q[n + np] = a2;
+        g_q[n + 2 * np] = a3;
+        g_E[n] = a4;
     }
 }
 
@@ -750,33 +777,30 @@ extern "C" int sod_full_rhs_vjp(int nelem, int npoin, int npoin_w, const int* co
     const size_t np = (size_t)npoin, ne = (size_t)nelem;
     // seeds and gradients as 6 / 5 contiguous fields of npoin
     void* buf = nullptr;
-    SOD_TRY(scratch(sizeof(double) * (np * 11 + ne * 64 + 8), &buf, 6));
+    SOD_TRY(scratch(sizeof(double) * (np * 6 + ne * 64 + 8), &buf, 6));
     double* lam = static_cast<double*>(buf);
-    double* grad = lam + 6 * np;
-    double* gml = grad + 5 * np;
+    double* gml = lam + 6 * np;
     SOD_HIP(hipMemcpyAsync(lam, dlm, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
     SOD_HIP(hipMemcpyAsync(lam + np, dlq, sizeof(double) * np * 3, hipMemcpyDeviceToDevice, s));
     SOD_HIP(hipMemcpyAsync(lam + 4 * np, dle, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
     SOD_HIP(hipMemcpyAsync(lam + 5 * np, dlt, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_zero_doubles, dim3(std::max(1, std::min(1024, grid_for((long)np * 5, 256)))), dim3(256), 0, s,
-                       (long)np * 5, grad);
     const sod_numerical_params& prm = rt().params;
     const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
+    SOD_PROF("full_rhs_vjp");   // the element kernel, the mu_e transpose and the node sums
     if (nelem > 0) {
-        SOD_PROF("full_rhs_vjp");
         if (lm.has_sgs)
             hipLaunchKernelGGL(k_stage_rhs_vjp<true>, dim3(grid_for(nelem, AWPB)), dim3(64 * AWPB), 0, s, nelem,
                                npoin, lm.conn, lm.He, lm.gpv, lm.mue, lm.musgs, dr, dq, dq + np, dq + 2 * np, dE, dmf,
-                               lam, grad, gml, sp);
+                               lam, lm.relem, gml, sp);
         else
             hipLaunchKernelGGL(k_stage_rhs_vjp<false>, dim3(grid_for(nelem, AWPB)), dim3(64 * AWPB), 0, s, nelem,
                                npoin, lm.conn, lm.He, lm.gpv, lm.mue, lm.musgs, dr, dq, dq + np, dq + 2 * np, dE, dmf,
-                               lam, grad, gml, sp);
+                               lam, lm.relem, gml, sp);
         hipLaunchKernelGGL(k_lane_to_ref_tiled, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, gml, dgmu);
     }
-    SOD_HIP(hipMemcpyAsync(dgr, grad, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
-    SOD_HIP(hipMemcpyAsync(dgq, grad + np, sizeof(double) * np * 3, hipMemcpyDeviceToDevice, s));
-    SOD_HIP(hipMemcpyAsync(dgE, grad + 4 * np, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+    if (npoin > 0)
+        hipLaunchKernelGGL(k_assemble_grad, dim3(std::min(2048, grid_for(npoin, 256))), dim3(256), 0, s, npoin,
+                           lm.inv_off, lm.inv_pos, lm.relem, dgr, dgq, dgE);
     SOD_HIP(hipGetLastError());
     return finish_call();
 }

@@ -100,26 +100,27 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))
 
     double(*S)[64] = sh[w];
     const int me = swz(i, j, k);
+    const double hk = h + kk;
     S[P_RHO][me] = r;
     S[P_U0][me] = u0;
     S[P_U1][me] = u1;
     S[P_U2][me] = u2;
     S[P_P][me] = p;
-    S[P_H][me] = h;
-    S[P_K][me] = kk;
+    S[P_H][me] = hk;
     S[P_ETA][me] = eta;
     S[P_T][me] = T;
     wave_sync();
 
     // ---- directional passes ---------------------------------------------
     // With DU_b = D * U^c_b (U^c the contravariant velocity formed with this
-    // node's He) and t_b = rho_b DU_b, the divergence of a flux a*u is the
-    // sum of a_b DU_b and that of a*rho*u the sum of a_b t_b; the advective
-    // derivative of a (of a*rho) is the sum of a_b DA (a_b sA) with
-    // DA = D * U^c(own) and sA = rho_b DA.
-    double divF0 = 0, divF1 = 0, divF2 = 0, divU0 = 0, divU1 = 0, divU2 = 0;
-    double divFe = 0, divFue = 0, divFk = 0, divFuk = 0, divFeta = 0, divQ = 0;
-    double Ah = 0, Arh = 0, Ak = 0, Ark = 0, Aq0 = 0, Aq1 = 0, Aq2 = 0, Aeta = 0;
+    // node's He), t_b = rho_b DU_b, DA_b = D * U^c(own) and sA_b = rho_b DA_b,
+    // the split-form sums of elem_convec.f90:414-434 factor per line point b:
+    //   energy   (h+k)_b (t_b + rho DU_b + rho DA_b + sA_b)   [div(rho h u), h div(rho u)...,
+    //                                                        the same with k, and the advective terms]
+    //   momentum u_a,b (t_b + rho DU_b + sA_b)
+    //   entropy  eta_b (DU_b + DA_b)
+    // (E := h in the energy flux, time_integ.f90:364; only h + k enters).
+    double divQ = 0, accE = 0, accEta = 0, accQ0 = 0, accQ1 = 0, accQ2 = 0;
     double gP[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, gR[3] = {0, 0, 0};
     double gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};   // gU[a][b] = du_a/dx_b
     const int own[3] = {i, j, k};
@@ -132,14 +133,14 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             const double Dc = sDn[o * 4 + d];
-            double rb, u0b, u1b, u2b, pb, hb, kb, etab, Tb;
+            double rb, u0b, u1b, u2b, pb, hkb, etab, Tb;
             if (d == 0) {
-                rb = r; u0b = u0; u1b = u1; u2b = u2; pb = p; hb = h; kb = kk; etab = eta; Tb = T;
+                rb = r; u0b = u0; u1b = u1; u2b = u2; pb = p; hkb = hk; etab = eta; Tb = T;
             } else {
                 const int bb = (o + d) & 3;
                 const int sl = c == 0 ? swz(bb, j, k) : (c == 1 ? swz(i, bb, k) : swz(i, j, bb));
                 rb = S[P_RHO][sl]; u0b = S[P_U0][sl]; u1b = S[P_U1][sl]; u2b = S[P_U2][sl];
-                pb = S[P_P][sl]; hb = S[P_H][sl]; kb = S[P_K][sl]; etab = S[P_ETA][sl]; Tb = S[P_T][sl];
+                pb = S[P_P][sl]; hkb = S[P_H][sl]; etab = S[P_ETA][sl]; Tb = S[P_T][sl];
             }
             dp += Dc * pb;
             du0 += Dc * u0b;
@@ -148,29 +149,15 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))
             dT += Dc * Tb;
             dr += Dc * rb;
             const double DU = Dc * (h0 * u0b + h1 * u1b + h2 * u2b);
-            const double t = rb * DU;
-            divF0 += u0b * t;
-            divF1 += u1b * t;
-            divF2 += u2b * t;
-            divU0 += u0b * DU;
-            divU1 += u1b * DU;
-            divU2 += u2b * DU;
-            divFe += hb * t;
-            divFue += hb * DU;
-            divFk += kb * t;
-            divFuk += kb * DU;
-            divFeta += etab * DU;
-            divQ += t;
             const double DA = Dc * Ua;
-            const double sA = rb * DA;
-            Ah += DA * hb;
-            Arh += sA * hb;
-            Ak += DA * kb;
-            Ark += sA * kb;
-            Aq0 += sA * u0b;
-            Aq1 += sA * u1b;
-            Aq2 += sA * u2b;
-            Aeta += DA * etab;
+            const double t = rb * DU;
+            const double Y = t + r * DU + rb * DA;        // t + rho DU + sA
+            accQ0 += u0b * Y;
+            accQ1 += u1b * Y;
+            accQ2 += u2b * Y;
+            accE += hkb * (Y + r * DA);
+            accEta += etab * (DU + DA);
+            divQ += t;
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -186,23 +173,18 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))
 
     // ---- convective residual (elem_convec.f90:414-434) -------------------
     const double uu[3] = {u0, u1, u2};
-    const double qq[3] = {r * u0, r * u1, r * u2};
     const double divU = gU[0][0] + gU[1][1] + gU[2][2];
     const double Arho = u0 * gR[0] + u1 * gR[1] + u2 * gR[2];
-    const double divF[3] = {divF0, divF1, divF2}, divFuu[3] = {divU0, divU1, divU2};
-    const double Aq[3] = {Aq0, Aq1, Aq2};
+    const double accQ[3] = {accQ0, accQ1, accQ2};
     double Rm = 0.5 * (divQ + r * divU) + 0.5 * Arho;
-    double Re = 0.25 * (divFe + h * divQ + (r * h) * divU + r * divFue) +
-                0.25 * (divFk + kk * divQ + (r * kk) * divU + r * divFuk) +
-                0.25 * (r * Ah + Arh + h * Arho) + 0.25 * (r * Ak + Ark + kk * Arho);
+    double Re = 0.25 * (accE + hk * (divQ + r * divU + Arho));
     double Rq[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const double Au = u0 * gU[a][0] + u1 * gU[a][1] + u2 * gU[a][2];
-        Rq[a] = 0.25 * (divF[a] + uu[a] * divQ + qq[a] * divU + r * divFuu[a]) +
-                0.25 * (uu[a] * Arho + r * Au + Aq[a]) + gP[a];
+        Rq[a] = 0.25 * (accQ[a] + uu[a] * (divQ + Arho) + (r * uu[a]) * divU + r * Au) + gP[a];
     }
-    const double Reta = 0.5 * (divFeta + eta * divU) + 0.5 * Aeta;
+    const double Reta = 0.5 * (accEta + eta * divU);
 
     // ---- viscous fluxes at this node (elem_diffu.f90:70-140) -------------
     const double mu_fgp = muf + r * mu_s;
@@ -1102,7 +1084,7 @@ int prepare_lane_mesh(int nelem, int npoin, int npoin_w, const int* connec, cons
     Prepared& P = *Pp;
     int has_sgs = 0;
     SOD_TRY(load_viscosities(P, nelem, mu_e, mu_sgs, &has_sgs));
-    *out = LaneMesh{P.conn, P.He, P.gpv, P.mue, P.musgs, has_sgs != 0};
+    *out = LaneMesh{P.conn, P.He, P.gpv, P.mue, P.musgs, has_sgs != 0, P.inv_off, P.inv_pos, P.relem};
     return SOD_OK;
 }
 
@@ -1122,9 +1104,9 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
     const size_t np = (size_t)npoin;
     int has_sgs = 0;
     SOD_TRY(load_viscosities(Pm, nelem, mu_e, mu_sgs, &has_sgs));
+    SOD_PROF("full_rhs");   // the element kernel and the node sums
     if (nelem > 0) {
         const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
-        SOD_PROF("full_rhs");
         if (has_sgs)
             hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, Pm.conn,
                                Pm.He, Pm.gpv, Pm.mue, Pm.musgs, rho, q, q + np, q + 2 * np, E, mu_factor, Pm.relem,

@@ -94,6 +94,10 @@ struct LaneMesh {
     const int* conn;
     const double *He, *gpv, *mue, *musgs;
     bool has_sgs;
+    // node -> (element, lane) positions and the element-node buffer their
+    // contributions are summed from
+    const int *inv_off, *inv_pos;
+    double* relem;
 };
 int prepare_lane_mesh(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
                       const double* He, const double* gpvol, const double* Ml, const double* mu_e,
