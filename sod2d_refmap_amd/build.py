@@ -87,6 +87,8 @@ FORTRAN = os.path.join(HERE, "fortran")
 FBUILD = os.path.join(FORTRAN, "_build")
 DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "shim_driver.f90")
 DRIVER = os.path.join(ROOT, "tests", "fortran", "_build", "shim_driver")
+IO_DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "io_driver.f90")
+IO_DRIVER = os.path.join(ROOT, "tests", "fortran", "_build", "io_driver")
 
 
 def build_fortran(verbose: bool = True) -> str:
@@ -107,8 +109,21 @@ def build_fortran(verbose: bool = True) -> str:
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"amdflang failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    # the HDF5 file module and its test driver (needs libsod2d_io.so)
+    io_obj = os.path.join(FBUILD, "mod_sod2d_io.o")
+    cmd = [FC, "-O2", "-fPIC", "-c", os.path.join(FORTRAN, "mod_sod2d_io.f90"), "-module-dir", FBUILD, "-o", io_obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if os.path.exists(IO_DRIVER_SRC) and os.path.exists(IO_LIB):
+        rpath = os.path.relpath(HERE, os.path.dirname(IO_DRIVER))
+        cmd = [FC, "-O2", IO_DRIVER_SRC, io_obj, "-I", FBUILD, "-module-dir", os.path.dirname(IO_DRIVER), "-L", HERE,
+               "-lsod2d_io", f"-Wl,-rpath,$ORIGIN/{rpath}", "-o", IO_DRIVER]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {obj} and {DRIVER}", flush=True)
+        print(f"built {obj}, {io_obj}, {DRIVER} and {IO_DRIVER}", flush=True)
     return obj
 
 
