@@ -128,11 +128,12 @@ def batch_speedup(count: int, steps: int):
     sols = sequential_solvers(members)
     log(f"ensemble of {count}: {E.nelem} elements, {E.npoin_w} working nodes "
         f"(setup {time.perf_counter() - t0:.1f}s)")
-    E.run(1)
+    # no time-level copies: the per-instance baseline makes none either
+    E.run(1, keep_levels=False)
     sequential_step(sols)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    E.run(steps)
+    E.run(steps, keep_levels=False)
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
     t0 = time.perf_counter()

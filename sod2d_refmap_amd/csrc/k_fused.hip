@@ -1213,6 +1213,9 @@ void launch_adapt_dt(int nelem, int npoin, const int* connec, const double* hele
                      const double* csound, double cfl_conv, int has_diff, double cfl_diff,
                      const double* mu_fluid, const double* mu_sgs, const double* rho, double* d_dt_bits);
 void launch_hmin(int nelem, int npoin, const int* connec, const double* helem, double* hmin);
+void launch_adapt_dt_nodes_inst(int npoin, const double* hmin, const double* u, const double* csound,
+                                double cfl_conv, int has_diff, double cfl_diff, const double* mu_fluid,
+                                const double* rho, const int* node_inst, int ninst, double* d_dt_bits);
 void launch_adapt_dt_inst(int nelem, int npoin, const int* connec, const double* helem, const double* u,
                           const double* csound, double cfl_conv, int has_diff, double cfl_diff,
                           const double* mu_fluid, const double* mu_sgs, const double* rho, const int* elem_inst,
@@ -1240,7 +1243,7 @@ struct sod_time_loop {
     double *rho, *u, *q, *pr, *E, *Tem, *e_int, *eta;   // full level arrays
     double* Ngp = nullptr;  // device copy of the library's Ngp
     double* sc = nullptr;   // dt[ninst], time[ninst], next dt[ninst] (folded CFL)
-    double* hmin = nullptr;  // per-node minimum element size (one mesh)
+    double* hmin = nullptr;  // per-node minimum element size
     int* ist = nullptr;     // istep, EK count
     int ninst = 1;          // > 1: a flattened ensemble described by B
     sod::BatchInfo B;
@@ -1319,6 +1322,9 @@ static void loop_enqueue_step(sod_time_loop* L, const LoopCfg& c) {
     if (!c.fixed_dt) {   // evalDt (CFDSolverBase.f90:1132-1141)
         if (fold)
             hipLaunchKernelGGL(k_copy_double, dim3(1), dim3(64), 0, s, L->sc + 2, L->sc);
+        else if (L->ninst > 1 && !c.has_sgs)
+            launch_adapt_dt_nodes_inst(a.npoin, L->hmin, a.u, a.csound, c.cfl_conv, c.has_diff, c.cfl_diff,
+                                       a.mu_fluid, a.rho, L->B.node_inst, L->ninst, L->sc);
         else if (L->ninst > 1)
             launch_adapt_dt_inst(a.nelem, a.npoin, a.connec, a.helem, a.u, a.csound, c.cfl_conv, c.has_diff,
                                  c.cfl_diff, a.mu_fluid, a.mu_sgs, a.rho, L->B.elem_inst, L->ninst, L->sc);
@@ -1410,7 +1416,7 @@ static int loop_create(sod_time_loop** out, int nelem, int npoin, int npoin_w, c
     if (rc == SOD_OK) {
         hipError_t e = hipMalloc(&L->Ngp, sizeof(double) * 64 * 64);
         if (e == hipSuccess) e = hipMalloc(&L->sc, sizeof(double) * (3 * (size_t)ninst + 8));
-        if (e == hipSuccess && ninst == 1) {
+        if (e == hipSuccess) {
             e = hipMalloc(&L->hmin, sizeof(double) * np);
             if (e == hipSuccess) {
                 launch_hmin(nelem, npoin, dc, dh, L->hmin);

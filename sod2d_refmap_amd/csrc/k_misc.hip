@@ -336,6 +336,44 @@ __global__ void k_zero_ints(int n, int* __restrict__ p) {
     for (int t = threadIdx.x; t < n; t += blockDim.x) p[t] = 0;
 }
 
+// adapt_dt_cfl of each instance of a flattened ensemble in node form
+// (node_dt, mu_sgs = 0): node n belongs to instance node_inst[n] and the
+// nodes of an instance are contiguous.  A wavefront takes 512 consecutive
+// nodes (8 per lane); when they all belong to one instance it folds their
+// minimum with one atomic, otherwise each lane folds its own.
+__global__ __launch_bounds__(256) void k_adapt_dt_nodes_inst(int npoin, const double* __restrict__ hmin,
+                                                             const double* __restrict__ u,
+                                                             const double* __restrict__ csound, double cfl_conv,
+                                                             int has_diff, double cfl_diff,
+                                                             const double* __restrict__ mu_fluid,
+                                                             const double* __restrict__ rho,
+                                                             const int* __restrict__ node_inst,
+                                                             unsigned long long* __restrict__ out_bits) {
+    const size_t np = (size_t)npoin;
+    const int lane = threadIdx.x & 63;
+    const long nwaves = ((long)npoin + 511) / 512;
+    for (long wv = (long)blockIdx.x * 4 + (threadIdx.x >> 6); wv < nwaves; wv += (long)gridDim.x * 4) {
+        const long n0 = wv * 512, n1 = min((long)npoin, n0 + 512);
+        const bool one = node_inst[n0] == node_inst[n1 - 1];
+        double best = HUGE_VAL;
+        for (long n = n0 + lane; n < n1; n += 64) {
+            const double h = hmin[n];
+            if (!(h < HUGE_VAL)) continue;   // a node no element holds
+            const double v = node_dt(h, u[n], u[n + np], u[n + 2 * np], csound[n], has_diff ? mu_fluid[n] : 0.0,
+                                     has_diff ? rho[n] : 1.0, cfl_conv, has_diff, cfl_diff);
+            if (one)
+                best = fmin(best, v);
+            else
+                atomicMin(out_bits + node_inst[n], (unsigned long long)__double_as_longlong(v));
+        }
+        if (one) {
+            best = wave_min(best);
+            if (lane == 0 && best < HUGE_VAL)
+                atomicMin(out_bits + node_inst[n0], (unsigned long long)__double_as_longlong(best));
+        }
+    }
+}
+
 // adapt_dt_cfl of each instance of a flattened ensemble: element e belongs to
 // instance elem_inst[e] and its dt folds into out_bits[elem_inst[e]].  Same
 // per-element arithmetic as k_adapt_dt; a tile whose 16 elements share one
@@ -713,6 +751,17 @@ int launch_adapt_dt_cached(int nelem, int npoin, const int* connec, const double
                            connec, helem, u, csound, cfl_conv, has_diff, cfl_diff, mu_fluid, mu_sgs, rho,
                            C.flags + 1, out);
     return SOD_OK;
+}
+
+void launch_adapt_dt_nodes_inst(int npoin, const double* hmin, const double* u, const double* csound,
+                                double cfl_conv, int has_diff, double cfl_diff, const double* mu_fluid,
+                                const double* rho, const int* node_inst, int ninst, double* d_dt_bits) {
+    hipStream_t s = rt().stream;
+    hipLaunchKernelGGL(k_set_doubles, dim3(1), dim3(256), 0, s, ninst, d_dt_bits, 100000000000000.0);
+    if (npoin > 0)
+        hipLaunchKernelGGL(k_adapt_dt_nodes_inst, dim3(std::min(2048, grid_for(npoin, 2048))), dim3(256), 0, s,
+                           npoin, hmin, u, csound, cfl_conv, has_diff, cfl_diff, mu_fluid, rho, node_inst,
+                           reinterpret_cast<unsigned long long*>(d_dt_bits));
 }
 
 void launch_adapt_dt_inst(int nelem, int npoin, const int* connec, const double* helem, const double* u,
