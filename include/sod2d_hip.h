@@ -324,6 +324,31 @@ int sod_time_loop_create_batched(sod_time_loop **loop, int ninst, const int *ele
 int sod_time_loop_run_batched(sod_time_loop *loop, int nsteps, double cfl_conv, double cfl_diff, int fixed_dt,
                               double *dt, double *time, int keep_levels);
 
+/* No reference counterpart: the configuration of the fused stage kernel used
+ * by sod_rk_4_main and the time loops.
+ *   mode 0 (default): 1 element per workgroup, residuals stored per element
+ *     node and summed per node in a fixed order (bit-reproducible);
+ *   mode 1: at the first step of each mesh the library times the
+ *     configurations with element-node stores (1, 2 or 4 elements per
+ *     workgroup) on that mesh and state, and keeps the fastest;
+ *   mode 2: as 1, with device-scope atomic scatter into node arrays (the
+ *     reference's assembly, elem_convec.f90:442-455; its summation order
+ *     varies from run to run) among the candidates.
+ * Every configuration gives the reference's results to the stated
+ * tolerances.  sod_full_rhs always uses the default. */
+int sod_set_autotune(int mode);
+/* No reference counterpart: pins one stage configuration (mode 3) instead of
+ * a tuned one: elems_per_workgroup 1, 2 or 4 with atomic_scatter 0, or 4
+ * with atomic_scatter 1.  sod_set_autotune returns to modes 0-2. */
+int sod_set_stage_config(int elems_per_workgroup, int atomic_scatter);
+/* The stage configuration of loop's mesh (loop NULL: the mesh most recently
+ * stepped through sod_rk_4_main): elements per workgroup, atomic scatter
+ * (0/1), the mode it was chosen under (0-2, or 3 when pinned) and the
+ * measured time of one stage in ms (0 when not tuned).  Any output pointer
+ * may be NULL. */
+int sod_stage_config(const sod_time_loop *loop, int *elems_per_workgroup, int *atomic_scatter, int *tuned_mode,
+                     double *stage_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,10 @@ _SIGS = {
     "sod_time_loop_create_batched": ([ctypes.POINTER(c_void_p), c_int, P, P, P] + [c_int] * 3 + [P] * 8 +
                                      [c_double] * 4 + [P] * 14),
     "sod_time_loop_run_batched": [c_void_p, c_int, c_double, c_double, c_int, P, P, c_int],
+    "sod_set_autotune": [c_int],
+    "sod_set_stage_config": [c_int, c_int],
+    "sod_stage_config": [c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                         ctypes.POINTER(c_double)],
 }
 
 _lib = None
@@ -112,6 +116,27 @@ def check(rc: int) -> None:
 
 def call(name: str, *args) -> None:
     check(getattr(lib(), name)(*args))
+
+
+def set_autotune(mode: int) -> None:
+    """How sod_rk_4_main and the time loops configure the stage kernel
+    (sod_set_autotune): 0 the default, 1 timed choice of the tile, 2 tile
+    and assembly strategy (atomic scatter among the candidates)."""
+    call("sod_set_autotune", int(mode))
+
+
+def set_stage_config(elems_per_workgroup: int, atomic_scatter: bool) -> None:
+    """Pins one stage configuration (sod_set_stage_config)."""
+    call("sod_set_stage_config", int(elems_per_workgroup), int(bool(atomic_scatter)))
+
+
+def stage_config(loop=None) -> dict:
+    """The stage configuration of a time loop's mesh, or (loop None) of the
+    mesh last stepped through sod_rk_4_main (sod_stage_config)."""
+    nw, at, md, ms = c_int(), c_int(), c_int(), c_double()
+    call("sod_stage_config", loop, ctypes.byref(nw), ctypes.byref(at), ctypes.byref(md), ctypes.byref(ms))
+    return {"elems_per_workgroup": nw.value, "atomic_scatter": bool(at.value), "mode": md.value,
+            "stage_ms": ms.value}
 
 
 def exported_symbols() -> list[str]:

@@ -32,7 +32,6 @@ namespace sod {
 
 namespace {
 
-constexpr int WPB = 4;   // wavefronts (elements) per workgroup
 constexpr int BLK = 256;
 
 // ------------------------------------------------------------ element kernel
@@ -42,15 +41,19 @@ constexpr int BLK = 256;
 // offset): sDn[o][d] = D[o][(o+d)&3] for the strong-form derivative and
 // sDt[o][d] = D[(o+d)&3][o] for the weak-form divergence.
 // Three wavefronts per SIMD (amdgpu_waves_per_eu(3)) measured faster than
-// the two the unconstrained register allocation gives, despite a few spills.
-template <bool HAS_SGS>
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_stage_rhs(
+// the two the unconstrained register allocation gives.
+// NW elements (wavefronts) per workgroup.  ATOMIC: the residuals are added
+// into six node arrays (stride rstride) with device-scope atomics, as the
+// reference scatters them (elem_convec.f90:442-455, elem_diffu.f90:183-193);
+// otherwise they are stored per element node into relem.
+template <bool HAS_SGS, int NW, bool ATOMIC>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_stage_rhs(
     int nelem, const int* __restrict__ conn, const double* __restrict__ He,
     const double* __restrict__ gpv, const double* __restrict__ mue, const double* __restrict__ musgs,
     const double* __restrict__ rho, const double* __restrict__ q0, const double* __restrict__ q1,
     const double* __restrict__ q2, const double* __restrict__ Ein, const double* __restrict__ mufac,
-    double* __restrict__ relem, StageParams P) {
-    __shared__ double sh[WPB][18][64];
+    double* __restrict__ relem, long rstride, StageParams P) {
+    __shared__ double sh[NW][18][64];
     // The derivative matrix is copied to LDS with a flat index first:
     // indexing the __constant__ 2-D table with a lane-dependent row gave
     // wrong addresses for this kernel with the current compiler.
@@ -69,7 +72,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))
     // [x*q + min(x, r8), ...) so that neighbouring elements, which share
     // nodes, meet in the same L2 for their gathers and atomics.
     const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int e = lb * WPB + w;
+    const int e = lb * NW + w;
     if (e >= nelem) return;   // uniform per wavefront; only wave-level sync below
     const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
     const size_t eb = (size_t)e * 64;
@@ -244,13 +247,20 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(3, 8))
     const double nu_e = P.c_rho * mu_e * ri;
     const double kappa = muf * P.CpPr + P.c_ener * mu_e * P.CpPr + P.Cp09 * r * mu_s;
 
-    // The residuals of this element at this node, stored in element-node
-    // order; k_stage_update and k_assemble_nodes sum each node's
-    // contributions (pull assembly: no atomics).
-    double2* o = reinterpret_cast<double2*>(relem + (eb + l) * NRES);
-    o[0] = make_double2(gv * Rm + nu_e * dD[0], gv * Rq[0] + dD[1]);
-    o[1] = make_double2(gv * Rq[1] + dD[2], gv * Rq[2] + dD[3]);
-    o[2] = make_double2(gv * Re + dD[4] + kappa * dD[5], gv * Reta);
+    const double res[NRES] = {gv * Rm + nu_e * dD[0], gv * Rq[0] + dD[1], gv * Rq[1] + dD[2],
+                              gv * Rq[2] + dD[3], gv * Re + dD[4] + kappa * dD[5], gv * Reta};
+    if (ATOMIC) {
+#pragma unroll
+        for (int f = 0; f < NRES; ++f) unsafeAtomicAdd(relem + f * rstride + n, res[f]);
+    } else {
+        // The residuals of this element at this node, stored in element-node
+        // order; k_stage_update and k_assemble_nodes sum each node's
+        // contributions (pull assembly: no atomics, a fixed summation order).
+        double2* o = reinterpret_cast<double2*>(relem + (eb + l) * NRES);
+        o[0] = make_double2(res[0], res[1]);
+        o[1] = make_double2(res[2], res[3]);
+        o[2] = make_double2(res[4], res[5]);
+    }
 }
 
 // Sum of the element-node residuals that belong to node n: the positions
@@ -308,6 +318,9 @@ struct StepState {
     // lane) positions they are summed over
     const double* relem;
     const int *inv_off, *inv_pos;
+    // node residual arrays of the atomic-scatter stage kernel (6 x npoin),
+    // zeroed again by the update that reads them
+    double* Rn;
     double *partial;   // reduction partials (RED_BLOCKS)
     // CFL time step of the next step folded into the final update (node_dt;
     // one mesh, mu_sgs = 0): per-node h_min, and the bits of the running
@@ -325,7 +338,7 @@ constexpr int RED_BLOCKS = 1024;
 // With BATCHED the nodes belong to independent instances (a flattened
 // ensemble): dt is d_dt[node_inst[n]] and the entropy statistics are formed
 // per instance by their own kernels.
-template <bool FINAL, bool BATCHED>
+template <bool FINAL, bool BATCHED, bool ATOMIC>
 __global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, const int* __restrict__ lpoin_w,
                                                       int s, const double* __restrict__ d_dt, double gamma,
                                                       double Rgas, StepState st,
@@ -340,7 +353,15 @@ __global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, co
         const double dt = BATCHED ? d_dt[node_inst[n]] : d_dt[0];
         const double ml = st.Ml[n];
         double Rf[NRES];
-        node_residuals(n, st.inv_off, st.inv_pos, st.relem, Rf);
+        if (ATOMIC) {
+#pragma unroll
+            for (int f = 0; f < NRES; ++f) {
+                Rf[f] = st.Rn[f * np + n];
+                st.Rn[f * np + n] = 0.0;
+            }
+        } else {
+            node_residuals(n, st.inv_off, st.inv_pos, st.relem, Rf);
+        }
 #pragma unroll
         for (int f = 0; f < NRES; ++f) Rf[f] = Rf[f] / ml;   // mod_solver.f90:32-36
         const double b = k_b[s];
@@ -702,6 +723,12 @@ struct Prepared {
     double* fpbuf = nullptr;
     double fp[4] = {0, 0, 0, 0};
     size_t work_doubles = 0;
+    // stage configuration (an entry of kStageCfgs) and how it was chosen:
+    // cfg_mode is the autotune mode it was tuned under (0: the default),
+    // cfg_ms the measured time of one stage with it
+    int cfg = 0, cfg_mode = 0;
+    double cfg_ms = 0.0;
+    double* rn = nullptr;   // 6 x npoin node residuals of the atomic-scatter configuration
     void release() {
         (void)hipFree(conn);
         (void)hipFree(He);
@@ -716,6 +743,7 @@ struct Prepared {
         (void)hipFree(flag);
         (void)hipFree(dscal);
         (void)hipFree(fpbuf);
+        (void)hipFree(rn);
         *this = Prepared();
     }
 };
@@ -952,12 +980,81 @@ struct DtFold {
     int has_diff;
 };
 
-static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_dt, bool has_sgs,
-                               bool mue_lane_out, const BatchInfo* B = nullptr, const DtFold* fold = nullptr) {
-    hipStream_t s = rt().stream;
-    const sod_numerical_params& prm = rt().params;
-    const int nelem = a.nelem, npoin = a.npoin, npoin_w = a.npoin_w;
-    const size_t np = (size_t)npoin;
+// ---------------------------------------------------- stage configurations
+// The stage kernel's tile (elements = wavefronts per workgroup) and assembly
+// strategy (element-node stores summed per node in a fixed order, or
+// device-scope atomic scatter into node arrays, whose summation order varies
+// from run to run).  Entry 0 is the default (the fastest at 64^3 TGV: a
+// one-wavefront workgroup releases its LDS as soon as its element is done);
+// sod_set_autotune lets the library time the others on the caller's mesh and
+// keep the fastest.
+struct StageCfg {
+    int nw;
+    bool atomic;
+};
+constexpr StageCfg kStageCfgs[] = {{1, false}, {2, false}, {4, false}, {4, true}};
+constexpr int kNumStageCfgs = sizeof(kStageCfgs) / sizeof(kStageCfgs[0]);
+constexpr int kPinned = 16;   // autotune mode kPinned + c: configuration c pinned
+
+template <bool HS, int NW, bool AT>
+static void launch_rhs_t(hipStream_t s, int nelem, const Prepared& P, const double* const in[6], long np,
+                         const StageParams& sp) {
+    hipLaunchKernelGGL((k_stage_rhs<HS, NW, AT>), dim3(grid_for(nelem, NW)), dim3(64 * NW), 0, s, nelem, P.conn,
+                       P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2], in[3], in[4], in[5],
+                       AT ? P.rn : P.relem, np, sp);
+}
+
+// stage kernel of configuration `cfg` on the state in[] = rho, q0, q1, q2, E, mu_factor
+template <bool HS>
+static void launch_rhs_hs(int cfg, hipStream_t s, int nelem, const Prepared& P, const double* const in[6], long np,
+                          const StageParams& sp) {
+    switch (cfg) {
+        case 1: launch_rhs_t<HS, 2, false>(s, nelem, P, in, np, sp); break;
+        case 2: launch_rhs_t<HS, 4, false>(s, nelem, P, in, np, sp); break;
+        case 3: launch_rhs_t<HS, 4, true>(s, nelem, P, in, np, sp); break;
+        default: launch_rhs_t<HS, 1, false>(s, nelem, P, in, np, sp); break;
+    }
+}
+static void launch_stage_rhs(int cfg, bool has_sgs, hipStream_t s, int nelem, const Prepared& P,
+                             const double* const in[6], long np, const StageParams& sp) {
+    if (has_sgs)
+        launch_rhs_hs<true>(cfg, s, nelem, P, in, np, sp);
+    else
+        launch_rhs_hs<false>(cfg, s, nelem, P, in, np, sp);
+}
+
+template <bool FINAL, bool BATCHED>
+static void launch_update_fb(bool atomic, dim3 grid, hipStream_t s, int npoin, int npoin_w, const int* lpoin_w,
+                             int stage, const double* d_dt, double gamma, double Rgas, const StepState& st,
+                             const int* ni) {
+    if (atomic)
+        hipLaunchKernelGGL((k_stage_update<FINAL, BATCHED, true>), grid, dim3(BLK), 0, s, npoin, npoin_w, lpoin_w,
+                           stage, d_dt, gamma, Rgas, st, ni);
+    else
+        hipLaunchKernelGGL((k_stage_update<FINAL, BATCHED, false>), grid, dim3(BLK), 0, s, npoin, npoin_w, lpoin_w,
+                           stage, d_dt, gamma, Rgas, st, ni);
+}
+static void launch_update(bool final_stage, bool batched, bool atomic, dim3 grid, hipStream_t s, int npoin,
+                          int npoin_w, const int* lpoin_w, int stage, const double* d_dt, double gamma,
+                          double Rgas, const StepState& st, const int* ni) {
+    if (final_stage) {
+        if (batched)
+            launch_update_fb<true, true>(atomic, grid, s, npoin, npoin_w, lpoin_w, stage, d_dt, gamma, Rgas, st, ni);
+        else
+            launch_update_fb<true, false>(atomic, grid, s, npoin, npoin_w, lpoin_w, stage, d_dt, gamma, Rgas, st, ni);
+    } else {
+        if (batched)
+            launch_update_fb<false, true>(atomic, grid, s, npoin, npoin_w, lpoin_w, stage, d_dt, gamma, Rgas, st, ni);
+        else
+            launch_update_fb<false, false>(atomic, grid, s, npoin, npoin_w, lpoin_w, stage, d_dt, gamma, Rgas, st,
+                                           ni);
+    }
+}
+
+// The node-kernel view of a step: the caller's level-2 arrays and the
+// prepared mesh's work arrays (no folded CFL step).
+static StepState step_state(Prepared& P, const FusedArgs& a) {
+    const size_t np = (size_t)a.npoin;
     StepState st;
     st.rho = a.rho;
     st.q0 = a.q;
@@ -980,10 +1077,21 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
     st.relem = P.relem;
     st.inv_off = P.inv_off;
     st.inv_pos = P.inv_pos;
+    st.Rn = P.rn;
     st.hmin = nullptr;
     st.dt_bits = nullptr;
     st.cfl_conv = st.cfl_diff = 0.0;
     st.has_diff = 0;
+    return st;
+}
+
+static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_dt, bool has_sgs,
+                               bool mue_lane_out, const BatchInfo* B = nullptr, const DtFold* fold = nullptr) {
+    hipStream_t s = rt().stream;
+    const sod_numerical_params& prm = rt().params;
+    const int nelem = a.nelem, npoin = a.npoin, npoin_w = a.npoin_w;
+    const size_t np = (size_t)npoin;
+    StepState st = step_state(P, a);
     if (fold != nullptr && B == nullptr && !has_sgs) {
         st.hmin = fold->hmin;
         st.dt_bits = reinterpret_cast<unsigned long long*>(fold->dt_bits);
@@ -1002,32 +1110,12 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
         }
         if (nelem > 0) {
             SOD_PROF("stage_rhs");
-            if (has_sgs)
-                hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem,
-                                   P.conn, P.He, P.gpv, P.mue, P.musgs, ir, i0, i1, i2, iE, a.mu_factor, P.relem,
-                                   sp);
-            else
-                hipLaunchKernelGGL(k_stage_rhs<false>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem,
-                                   P.conn, P.He, P.gpv, P.mue, P.musgs, ir, i0, i1, i2, iE, a.mu_factor, P.relem,
-                                   sp);
+            const double* const in[6] = {ir, i0, i1, i2, iE, a.mu_factor};
+            launch_stage_rhs(P.cfg, has_sgs, s, nelem, P, in, (long)np, sp);
         }
         SOD_PROF("stage_update");
-        const int* ni = B ? B->node_inst : nullptr;
-        if (stage < 3) {
-            if (B)
-                hipLaunchKernelGGL((k_stage_update<false, true>), dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w,
-                                   a.lpoin_w, stage, d_dt, a.gamma_gas, a.Rgas, st, ni);
-            else
-                hipLaunchKernelGGL((k_stage_update<false, false>), dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w,
-                                   a.lpoin_w, stage, d_dt, a.gamma_gas, a.Rgas, st, ni);
-        } else {
-            if (B)
-                hipLaunchKernelGGL((k_stage_update<true, true>), dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w,
-                                   a.lpoin_w, stage, d_dt, a.gamma_gas, a.Rgas, st, ni);
-            else
-                hipLaunchKernelGGL((k_stage_update<true, false>), dim3(ugrid), dim3(BLK), 0, s, npoin, npoin_w,
-                                   a.lpoin_w, stage, d_dt, a.gamma_gas, a.Rgas, st, ni);
-        }
+        launch_update(stage == 3, B != nullptr, kStageCfgs[P.cfg].atomic, dim3(ugrid), s, npoin, npoin_w, a.lpoin_w,
+                      stage, d_dt, a.gamma_gas, a.Rgas, st, B ? B->node_inst : nullptr);
     }
     if (P.nslave > 0)
         hipLaunchKernelGGL(k_slave_sutherland, dim3(grid_for(P.nslave, BLK)), dim3(BLK), 0, s, P.nslave,
@@ -1058,6 +1146,86 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
     }
 }
 
+static_assert(kNumStageCfgs == 4 && kStageCfgs[0].nw == 1 && !kStageCfgs[0].atomic && kStageCfgs[1].nw == 2 &&
+                  !kStageCfgs[1].atomic && kStageCfgs[2].nw == 4 && !kStageCfgs[2].atomic &&
+                  kStageCfgs[3].nw == 4 && kStageCfgs[3].atomic,
+              "launch_rhs_hs dispatches on these entries");
+
+// Sets P.cfg for the autotune mode (sod_set_autotune): 0 keeps the default
+// configuration; 1 takes the fastest configuration with element-node stores;
+// 2 the fastest of all, atomic scatter included; kPinned + c pins entry c
+// (sod_set_stage_config).  Each candidate runs the
+// first stage of a step on the caller's level-2 state -- the stage kernel and
+// the node update that sums its residuals -- once to warm up and three times
+// timed with HIP events on the library stream; the smallest time wins.  Only
+// library work arrays are written.  Runs once per prepared mesh and mode,
+// never inside a graph capture.
+static int choose_stage_cfg(Prepared& P, const FusedArgs& a, bool has_sgs, const double* d_dt,
+                            const BatchInfo* B) {
+    const int mode = rt().autotune;
+    if (P.cfg_mode == mode) return SOD_OK;
+    hipStream_t s = rt().stream;
+    const size_t np = (size_t)a.npoin;
+    if (mode == 0 || mode >= kPinned) {
+        const int c = mode == 0 ? 0 : mode - kPinned;
+        if (kStageCfgs[c].atomic && P.rn == nullptr) {
+            SOD_HIP(hipMalloc(&P.rn, sizeof(double) * NRES * np + 8));
+            SOD_HIP(hipMemsetAsync(P.rn, 0, sizeof(double) * NRES * np, s));
+        }
+        P.cfg = c;
+        P.cfg_mode = mode;
+        P.cfg_ms = 0.0;
+        return SOD_OK;
+    }
+    const bool try_atomic = mode >= 2;
+    if (try_atomic && P.rn == nullptr) {
+        SOD_HIP(hipMalloc(&P.rn, sizeof(double) * NRES * np + 8));
+        SOD_HIP(hipMemsetAsync(P.rn, 0, sizeof(double) * NRES * np, s));
+    }
+    const StepState st = step_state(P, a);
+    const sod_numerical_params& prm = rt().params;
+    const StageParams sp = make_stage_params(a.gamma_gas, a.Rgas, a.Cp, a.Prt, prm.c_rho, prm.c_ener);
+    const int ugrid = std::min(RED_BLOCKS, std::max(1, grid_for(a.npoin_w, BLK)));
+    const double* const in[6] = {a.rho, a.q, a.q + np, a.q + 2 * np, a.E, a.mu_factor};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SOD_HIP(hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    double best = HUGE_VAL;
+    int best_cfg = 0;
+    for (int c = 0; c < kNumStageCfgs && err == hipSuccess; ++c) {
+        if (kStageCfgs[c].atomic && !try_atomic) continue;
+        double tmin = HUGE_VAL;
+        for (int rep = 0; rep < 4 && err == hipSuccess; ++rep) {
+            err = hipEventRecord(e0, s);
+            if (a.nelem > 0) launch_stage_rhs(c, has_sgs, s, a.nelem, P, in, (long)np, sp);
+            launch_update(false, B != nullptr, kStageCfgs[c].atomic, dim3(ugrid), s, a.npoin, a.npoin_w, a.lpoin_w,
+                          0, d_dt, a.gamma_gas, a.Rgas, st, B ? B->node_inst : nullptr);
+            if (err == hipSuccess) err = hipGetLastError();
+            if (err == hipSuccess) err = hipEventRecord(e1, s);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0) tmin = std::min(tmin, (double)ms);
+        }
+        if (tmin < best) {
+            best = tmin;
+            best_cfg = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (err != hipSuccess) return fail(SOD_ERR_HIP, "stage autotuning: %s", hipGetErrorString(err));
+    P.cfg = best_cfg;
+    P.cfg_mode = mode;
+    P.cfg_ms = best;
+    if (!kStageCfgs[best_cfg].atomic && P.rn != nullptr) {
+        SOD_HIP(hipStreamSynchronize(s));
+        (void)hipFree(P.rn);
+        P.rn = nullptr;
+    }
+    return SOD_OK;
+}
+
 int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
                    const double* He, const double* gpvol, const double* Ml, const double* helem,
                    const double* helem_l, double dt, double Rgas, double gamma_gas, double Cp, double Prt,
@@ -1073,6 +1241,7 @@ int rk4_step_fused(int nelem, int npoin, int npoin_w, const int* connec, const i
                       gamma_gas, Cp, Prt, rho, u, q, pr, E, Tem, e_int, eta, csound, machno, mu_fluid, mu_e,
                       mue_l, mu_sgs};
     hipLaunchKernelGGL(k_set_double_f, dim3(1), dim3(64), 0, rt().stream, P.dscal, dt);
+    SOD_TRY(choose_stage_cfg(P, a, has_sgs != 0, P.dscal, nullptr));
     enqueue_fused_step(P, a, P.dscal, has_sgs != 0, false);
     SOD_HIP(hipGetLastError());
     return SOD_OK;
@@ -1109,14 +1278,9 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
     SOD_PROF("full_rhs");   // the element kernel and the node sums
     if (nelem > 0) {
         const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
-        if (has_sgs)
-            hipLaunchKernelGGL(k_stage_rhs<true>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, Pm.conn,
-                               Pm.He, Pm.gpv, Pm.mue, Pm.musgs, rho, q, q + np, q + 2 * np, E, mu_factor, Pm.relem,
-                               sp);
-        else
-            hipLaunchKernelGGL(k_stage_rhs<false>, dim3(grid_for(nelem, WPB)), dim3(64 * WPB), 0, s, nelem, Pm.conn,
-                               Pm.He, Pm.gpv, Pm.mue, Pm.musgs, rho, q, q + np, q + 2 * np, E, mu_factor, Pm.relem,
-                               sp);
+        // the default configuration (element-node stores, summed below)
+        const double* const in[6] = {rho, q, q + np, q + 2 * np, E, mu_factor};
+        launch_stage_rhs(0, has_sgs != 0, s, nelem, Pm, in, (long)np, sp);
     }
     if (npoin > 0)
         hipLaunchKernelGGL(k_assemble_nodes, dim3(std::min(2048, grid_for(npoin, 256))), dim3(256), 0, s, npoin,
@@ -1228,9 +1392,10 @@ struct LoopCfg {
     bool has_sgs = false, fixed_dt = false, keep_levels = false;
     double cfl_conv = 0, cfl_diff = 0, rho0 = 0;
     int log_first = 0, log_every = 0, ek_cap = 0, has_diff = 0;
+    int stage_cfg = 0;
     sod_numerical_params params{};
     bool operator==(const LoopCfg& o) const {
-        return valid == o.valid && has_sgs == o.has_sgs && fixed_dt == o.fixed_dt &&
+        return valid == o.valid && has_sgs == o.has_sgs && fixed_dt == o.fixed_dt && stage_cfg == o.stage_cfg &&
                keep_levels == o.keep_levels && cfl_conv == o.cfl_conv && cfl_diff == o.cfl_diff &&
                rho0 == o.rho0 && log_first == o.log_first && log_every == o.log_every && ek_cap == o.ek_cap &&
                has_diff == o.has_diff && std::memcmp(&params, &o.params, sizeof(params)) == 0;
@@ -1549,8 +1714,9 @@ static int loop_run(sod_time_loop* L, int istep0, int nsteps, double cfl_conv, d
     SOD_HIP(hipStreamSynchronize(saved));
     rt().stream = L->stream;
     const bool prof = prof_set(false);
-    int rc = SOD_OK;
-    if (!L->exec || !(c == L->cfg)) {
+    int rc = choose_stage_cfg(L->P, L->a, c.has_sgs, L->sc, ni > 1 ? &L->B : nullptr);
+    c.stage_cfg = L->P.cfg;
+    if (rc == SOD_OK && (!L->exec || !(c == L->cfg))) {
         loop_free_graph(L);
         hipError_t e = hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
@@ -1614,5 +1780,43 @@ extern "C" int sod_time_loop_destroy(sod_time_loop* L) {
     (void)hipFree(L->inst_stats);
     L->P.release();
     delete L;
+    return SOD_OK;
+}
+
+extern "C" int sod_set_autotune(int mode) {
+    if (mode < 0 || mode > 2)
+        return fail(SOD_ERR_ARG, "sod_set_autotune: mode %d (0: default, 1: tiles, 2: tiles and atomic scatter)",
+                    mode);
+    rt().autotune = mode;
+    return SOD_OK;
+}
+
+extern "C" int sod_set_stage_config(int elems_per_workgroup, int atomic_scatter) {
+    for (int c = 0; c < kNumStageCfgs; ++c)
+        if (kStageCfgs[c].nw == elems_per_workgroup && kStageCfgs[c].atomic == (atomic_scatter != 0)) {
+            rt().autotune = kPinned + c;
+            return SOD_OK;
+        }
+    return fail(SOD_ERR_ARG, "sod_set_stage_config: no configuration with %d elements per workgroup and %s",
+                elems_per_workgroup, atomic_scatter ? "atomic scatter" : "element-node stores");
+}
+
+extern "C" int sod_stage_config(const sod_time_loop* loop, int* elems_per_workgroup, int* atomic_scatter,
+                                int* tuned_mode, double* stage_ms) {
+    const Prepared* P = nullptr;
+    if (loop != nullptr) {
+        P = &loop->P;
+    } else {
+        const PrepSlot* last = nullptr;
+        for (const auto& sl : g_slots)
+            if (sl.P.nelem >= 0 && (last == nullptr || sl.used > last->used)) last = &sl;
+        if (last != nullptr) P = &last->P;
+    }
+    if (P == nullptr) return fail(SOD_ERR_ARG, "sod_stage_config: no mesh has been stepped yet");
+    const StageCfg& c = kStageCfgs[P->cfg];
+    if (elems_per_workgroup) *elems_per_workgroup = c.nw;
+    if (atomic_scatter) *atomic_scatter = c.atomic ? 1 : 0;
+    if (tuned_mode) *tuned_mode = P->cfg_mode >= kPinned ? 3 : P->cfg_mode;
+    if (stage_ms) *stage_ms = P->cfg_ms;
     return SOD_OK;
 }

@@ -99,6 +99,7 @@ struct Runtime {
     hipStream_t stream = nullptr;
     bool sync_on_return = true;
     sod_numerical_params params{};
+    int autotune = 0;   // stage configuration choice (sod_set_autotune)
 };
 Runtime& rt();
 int ensure_init();

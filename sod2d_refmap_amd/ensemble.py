@@ -121,6 +121,12 @@ class Ensemble:
                   dt.ctypes.data, tm.ctypes.data, int(keep_levels))
         self.dt, self.time = dt, tm
 
+    def stage_config(self) -> dict:
+        """The stage configuration the ensemble's loop runs with (_lib.stage_config)."""
+        if self._loop is None:
+            raise RuntimeError("the ensemble has not been run yet")
+        return _lib.stage_config(self._loop)
+
     def member_state(self, i: int) -> dict:
         """Level-2 state arrays and element-node viscosities of member i."""
         n0, n1 = self.noff[i], self.noff[i + 1]

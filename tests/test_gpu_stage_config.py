@@ -1,0 +1,158 @@
+"""Stage-kernel configurations (sod_set_autotune, sod_set_stage_config): 1, 2
+or 4 elements per workgroup with element-node stores, and 4 with atomic
+scatter into node arrays.  Every configuration steps a curvilinear mesh with
+permuted elements to the oracle's tolerances; the autotuner picks one of the
+candidates of its mode and reports it; the device time loop and a batched
+ensemble run with a pinned or tuned configuration and match the default
+eager steps.  Every test returns the library to the default configuration."""
+import numpy as np
+import pytest
+
+import oracle
+from _util import mesh_geometry, rel_l2
+from sod2d_refmap_amd import _lib, cases, mesh as pmesh
+from sod2d_refmap_amd.device import numerical_params, set_numerical_params
+
+pytestmark = pytest.mark.gpu
+
+T = oracle.hex64_tables()
+CFGS = [(4, False), (2, False), (1, False), (4, True)]
+KEYS = ("rho", "q", "E", "u", "pr", "Tem", "eta", "csound", "machno", "mu_fluid", "mu_e", "mue_l")
+
+
+@pytest.fixture
+def default_after():
+    yield
+    _lib.set_autotune(0)
+
+
+def _case(n=3, perm_seed=11):
+    m = pmesh.periodic_box(n, curved_amp=0.05, perm_seed=perm_seed)
+    gas = cases.GasParams()
+    s = cases.tgv_initial_state(m, gas, perturb_seed=7, perturb_amp=5e-2)
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    return m, gas, s
+
+
+def _oracle_steps(m, g, gas, s, dt, nsteps):
+    s = {k: v.copy(order="F") for k, v in s.items()}
+    p = oracle.default_params(Rgas_inf=gas.Rgas)
+    out = []
+    for _ in range(nsteps):
+        oracle.rk_4_main(m.connec, m.lpoin_w, g["He"], g["gpvol"], T, g["Ml"], g["helem"], g["helem_l"], dt,
+                         gas.Rgas, gas.gamma_gas, gas.Cp, gas.Prt, s["mu_factor"], s, p)
+        out.append({k: v.copy() for k, v in s.items()})
+    return out
+
+
+@pytest.mark.parametrize("nw,atomic", CFGS, ids=[f"nw{n}{'_atomic' if a else ''}" for n, a in CFGS])
+def test_each_configuration_matches_oracle(gpu, default_after, nw, atomic):
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    m, gas, s = _case()
+    g = mesh_geometry(m)
+    dt = oracle.adapt_dt_cfl(m.connec, g["helem"], s["u"], s["csound"], 0.5, 0.5, s["mu_fluid"],
+                             np.zeros((m.nelem, 64), order="F"), s["rho"])
+    ref = _oracle_steps(m, g, gas, s, dt, 10)
+    _lib.set_stage_config(nw, atomic)
+    sol = PeriodicSolver(m, gas, s)
+    print(f"[stage config] {nw} elements per workgroup, atomic={atomic}", flush=True)
+    for step in range(10):
+        sol.rk_4_main(dt)
+        if step in (0, 9):
+            out = sol.state()
+            tol = 1e-12 if step == 0 else 1e-10
+            for k in KEYS:
+                err = rel_l2(out[k], ref[step][k])
+                assert err < tol, (step, k, err)
+    c = _lib.stage_config()
+    assert (c["elems_per_workgroup"], c["atomic_scatter"], c["mode"]) == (nw, atomic, 3)
+    sol.close()
+
+
+def test_unknown_configuration_is_rejected(gpu, default_after):
+    with pytest.raises(_lib.Sod2dError, match="no configuration"):
+        _lib.set_stage_config(8, False)
+    with pytest.raises(_lib.Sod2dError, match="mode"):
+        _lib.set_autotune(3)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_autotuned_step_matches_oracle(gpu, default_after, mode):
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    m, gas, s = _case(perm_seed=5)
+    g = mesh_geometry(m)
+    dt = oracle.adapt_dt_cfl(m.connec, g["helem"], s["u"], s["csound"], 0.5, 0.5, s["mu_fluid"],
+                             np.zeros((m.nelem, 64), order="F"), s["rho"])
+    ref = _oracle_steps(m, g, gas, s, dt, 2)
+    _lib.set_autotune(mode)
+    sol = PeriodicSolver(m, gas, s)
+    for _ in range(2):
+        sol.rk_4_main(dt)
+    c = _lib.stage_config()
+    print(f"[autotune {mode}] chose {c}", flush=True)
+    assert c["mode"] == mode and c["stage_ms"] > 0.0
+    assert (c["elems_per_workgroup"], c["atomic_scatter"]) in (CFGS if mode == 2 else CFGS[:3])
+    out = sol.state()
+    for k in KEYS:
+        err = rel_l2(out[k], ref[1][k])
+        assert err < 1e-11, (k, err)
+    sol.close()
+
+
+def test_time_loop_across_configurations(gpu, default_after):
+    """The loop re-captures its step when the configuration changes between
+    runs: 2 steps pinned to 1 element per workgroup, 2 with atomic scatter,
+    2 tuned with every candidate, against 6 default eager steps."""
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    m, gas, s = _case(perm_seed=13)
+    A = PeriodicSolver(m, gas, s)
+    B = PeriodicSolver(m, gas, s)
+    A.adapt_dt()
+    B.adapt_dt()
+    for _ in range(6):
+        A.rk_4_main(A.dt)
+        A.time += A.dt
+        A.adapt_dt()
+    for setting in ((1, False), (4, True), "tuned"):
+        print(f"[time loop] configuration {setting}", flush=True)
+        if setting == "tuned":
+            _lib.set_autotune(2)
+        else:
+            _lib.set_stage_config(*setting)
+        B.run_device(2, keep_levels=False)
+        c = _lib.stage_config(B._loop)
+        if setting != "tuned":
+            assert (c["elems_per_workgroup"], c["atomic_scatter"]) == setting
+        else:
+            assert c["mode"] == 2 and c["stage_ms"] > 0.0
+    _lib.set_autotune(0)
+    sa, sb = A.state(), B.state()
+    for k in KEYS:
+        err = rel_l2(sb[k], sa[k])
+        assert err < 1e-11, (k, err)
+    assert abs(B.dt / A.dt - 1) < 1e-11
+    A.close()
+    B.close()
+
+
+def test_ensemble_with_atomic_scatter(gpu, default_after):
+    from sod2d_refmap_amd.ensemble import Ensemble
+    members = []
+    for n, amp, perm, Re, Ma, seed in [(2, 0.0, None, 800.0, 0.1, 1), (3, 0.05, 9, 1600.0, 0.1, 2)]:
+        m = pmesh.periodic_box(n, curved_amp=amp, perm_seed=perm)
+        gas = cases.GasParams(Re=Re, Mach=Ma)
+        members.append((m, gas, cases.tgv_initial_state(m, gas, perturb_seed=seed, perturb_amp=1e-2)))
+    D = Ensemble(members)
+    D.run(3)
+    _lib.set_stage_config(4, True)
+    E = Ensemble(members)
+    E.run(3)
+    c = E.stage_config()
+    assert (c["elems_per_workgroup"], c["atomic_scatter"]) == (4, True)
+    for i in range(len(members)):
+        got, ref = E.member_state(i), D.member_state(i)
+        for k in KEYS:
+            err = rel_l2(got[k], ref[k])
+            assert err < 1e-12, (i, k, err)
+    D.close()
+    E.close()
