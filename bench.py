@@ -262,6 +262,8 @@ def main():
                   "alg_GBs": round(stage_gbs, 1), "frac_of_hbm_peak": round(stage_gbs / HBM_PEAK_GBS, 4)},
         "ek_final": ek,
     }
+    # the stage-kernel configuration the timed steps ran with (sod_stage_config)
+    out["stage_config"] = _lib.stage_config()
     traffic = measured_traffic(dom, n)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic

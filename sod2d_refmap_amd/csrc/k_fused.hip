@@ -23,6 +23,7 @@
 // operator as the reference's (He applied to the xi-derivative of each product)
 // rearranged by linearity.
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -674,6 +675,9 @@ __global__ __launch_bounds__(256) void k_fingerprint(int nelem, int npoin_w, con
     __shared__ double s[4][256];
     const int t = threadIdx.x;
     double a = 0, b = 0, c = 0, d = 0;
+    // unrolled so that the 16 rounds' loads are issued together (the sums
+    // keep their order)
+#pragma unroll
     for (int smp = 0; smp < 16; ++smp) {
         const int e = (int)(((long)nelem * smp + (t >> 4)) / 16) % nelem;
         const int nd = t & 63;
@@ -787,8 +791,17 @@ __global__ __launch_bounds__(256) void k_check_conn(long n, const int* __restric
     if (b) *bad = 1;
 }
 
+// The inverse connectivity holds element-node positions e*64 + lane as int32.
+static int check_mesh_size(int nelem) {
+    if ((long long)nelem * 64 > (long long)INT_MAX)
+        return fail(SOD_ERR_ARG, "nelem %d: at most %d elements per mesh (32-bit element-node positions)", nelem,
+                    INT_MAX / 64);
+    return SOD_OK;
+}
+
 static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
                    const double* He, const double* gpvol, const double* Ml) {
+    SOD_TRY(check_mesh_size(nelem));
     const void* key[6] = {connec, lpoin_w, He, gpvol, Ml, nullptr};
     bool same = P.nelem == nelem && P.npoin == npoin && P.npoin_w == npoin_w;
     for (int t = 0; t < 5 && same; ++t) same = P.key[t] == key[t];
@@ -880,6 +893,7 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
 // rebuilt by prepare() as needed).
 static int prepare_cached(Prepared** out, int nelem, int npoin, int npoin_w, const int* connec,
                           const int* lpoin_w, const double* He, const double* gpvol, const double* Ml) {
+    SOD_TRY(check_mesh_size(nelem));
     const void* key[5] = {connec, lpoin_w, He, gpvol, Ml};
     PrepSlot* hit = nullptr;
     for (auto& sl : g_slots) {

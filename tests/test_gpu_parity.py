@@ -26,6 +26,8 @@ MESHES = {
     "straight2": dict(n=2),
     "straight4": dict(n=4),
     "curved3_perm": dict(n=3, curved_amp=0.05, perm_seed=11),
+    # 3 x 2 x 4 elements of aspect ratio up to 4, curved and permuted
+    "aniso324_curved_perm": dict(n=(3, 2, 4), L=(2 * np.pi, 4 * np.pi, 2 * np.pi), curved_amp=0.03, perm_seed=3),
 }
 
 
@@ -312,3 +314,14 @@ def test_present_table_host_pointers(gpu):
     assert rel_l2(arrs["R"], ref) < TOL1
     for a in arrs.values():
         _lib.call("sod_exit_data", a.ctypes.data, 0)
+
+
+def test_mesh_size_limit_is_rejected(gpu):
+    """Element-node positions are 32-bit: a mesh of 2^31 / 64 elements or more
+    is refused before anything is allocated or launched."""
+    z = torch.zeros(64, dtype=torch.float64, device="cuda")
+    zi = torch.ones(64, dtype=torch.int32, device="cuda")
+    ne = (1 << 31) // 64
+    with pytest.raises(_lib.Sod2dError, match="at most 33554431 elements"):
+        _lib.call("sod_full_rhs", ne, 8, 8, zi.data_ptr(), zi.data_ptr(), *[z.data_ptr()] * 9, 287.0, 1.4, 1004.0,
+                  0.71, *[z.data_ptr()] * 4)
