@@ -22,6 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 vector
 
 
 def log(*a):
@@ -78,6 +79,25 @@ def measured_traffic(kernel: str, n: int):
         if line.startswith("bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def measured_flops_per_wave(kernel: str, n: int):
+    """FP64 operations per wavefront (one element) of `kernel`: 2 per FMA,
+    1 per add or multiply, from the SQ instruction counters of the committed
+    rocprofv3 summary for this workload, when one exists."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_tgv{n}_sq_counters.txt")
+    if not os.path.exists(path):
+        return None
+    per_wave = {}
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0].startswith("SQ_INSTS_VALU_"):
+            per_wave[f[0]] = float(f[2])
+    try:
+        return 64 * (2 * per_wave["SQ_INSTS_VALU_FMA_F64"] + per_wave["SQ_INSTS_VALU_ADD_F64"] +
+                     per_wave["SQ_INSTS_VALU_MUL_F64"])
+    except KeyError:
+        return None
 
 
 def adjoint_ratio(sol, m, reps: int = 5):
@@ -267,6 +287,14 @@ def main():
     traffic = measured_traffic(dom, n)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
+    # the stage kernel is co-bound by FP64 issue: the same launch time against
+    # the FP64 vector peak, with the operation count the SQ counters give
+    fpw = measured_flops_per_wave(dom, n)
+    if fpw is not None:
+        tf = fpw * Ne / avg_s / 1e12
+        out["roofline_fp64"] = {"bound": "fp64-valu", "achieved": round(tf, 2), "peak": FP64_PEAK_TFLOPS,
+                                "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFLOPS, 4),
+                                "flop_per_element": fpw}
     # the same steps through the device time loop (graph-captured step, dt on
     # the device, no time-level copies: the same work as the eager steps):
     # reported beside the eager figure
