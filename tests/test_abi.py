@@ -76,3 +76,20 @@ def test_binding_argument_counts_match_header():
         if n != len(args):
             bad.append((name, n, len(args)))
     assert not bad, bad
+
+
+def test_stage_configuration_arguments_are_checked():
+    """sod_set_autotune / sod_set_stage_config only record a setting (no
+    device work): unknown modes and configurations are refused with a
+    message, and the default is restored."""
+    import pytest
+    for bad in (-1, 3, 16):
+        with pytest.raises(_lib.Sod2dError, match="mode"):
+            _lib.set_autotune(bad)
+    for nw, atomic in ((8, False), (3, False), (1, True), (2, True)):
+        with pytest.raises(_lib.Sod2dError, match="no configuration"):
+            _lib.set_stage_config(nw, atomic)
+    for nw, atomic in ((1, False), (2, False), (4, False), (4, True)):
+        _lib.set_stage_config(nw, atomic)
+    for mode in (1, 2, 0):
+        _lib.set_autotune(mode)
