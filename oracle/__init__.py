@@ -1,8 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
-by the cpu_baseline leg of bench.py, as the checker.  The product package
-(sod2d_refmap_amd) never imports it.
+by the cpu_baseline leg of bench.py, as the reference the HIP results are
+compared with.  The product package (sod2d_refmap_amd) never imports it.
 
 All arrays use the reference's Fortran layouts (numpy order='F') and its
 1-based index values; see oracle/sod2d_oracle.h.

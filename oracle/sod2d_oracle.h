@@ -2,8 +2,9 @@
  * sod2d_oracle.h -- CPU restatement of SOD2D's explicit compressible
  * spectral-element right-hand side (HEX64, p = 3, FP64).
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker used by tests/,
- * __graft_entry__.smoke() and the cpu_baseline leg of bench.py.  It is never
+ * TEST INFRASTRUCTURE ONLY.  This library is what tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py compare the
+ * HIP results with.  It is never
  * linked into, called by, or shipped with the product library
  * (sod2d_refmap_amd/, libsod2d_hip.so).
  *
