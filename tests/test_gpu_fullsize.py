@@ -1,0 +1,239 @@
+"""Parity at full size: the 64^3 Taylor-Green case and the 96^3 curvilinear
+case (elements permuted, Taylor-Green plus seeded Fourier modes |k| <= 8 of
+amplitude 1e-2, Re 3200, Mach 0.2), through checks whose cost does not grow
+with the mesh:
+
+- the assembled right-hand side (sod_full_rhs) at the 8 interior nodes of
+  sampled elements, which only their own element touches, equals the
+  oracle's right-hand side of that element alone (relative L2 <= 1e-12 over
+  the samples, per field).  The states carry seeded Fourier modes: the
+  unperturbed Taylor-Green state is close to equilibrium, and its residuals
+  are 30 to 2000 times smaller than the terms they are formed from (the
+  oracle on the 8^3 case), so a comparison there would measure that
+  cancellation rather than the kernels;
+- momentum is conserved over RK4 steps (split-form convection and weak-form
+  viscous terms are conservative), and mass too with the entropy viscosity
+  off (cglob = 0), as the oracle shows on small meshes;
+- steps from one saved state are bit-reproducible.
+
+The initial states follow sod2d_refmap_amd.cases.tgv_initial_state; the
+element-wise part is evaluated on the device from the solver's coordinates
+(the host version takes minutes at 96^3)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from _util import rel_l2, stage_fields
+from sod2d_refmap_amd import _lib, cases, mesh as pmesh
+from sod2d_refmap_amd.mesh import hex64_tables
+from sod2d_refmap_amd.device import numerical_params, ptr, set_numerical_params
+
+pytestmark = pytest.mark.gpu
+
+T = oracle.hex64_tables()
+CASES = {
+    "tgv64": dict(n=64, curved_amp=0.0, perm_seed=None, Re=1600.0, Mach=0.1, modes_seed=None),
+    "curv96_perm_modes": dict(n=96, curved_amp=0.05, perm_seed=1, Re=3200.0, Mach=0.2, modes_seed=2),
+}
+INTERIOR = np.nonzero((T.gmshAtoI >= 3) & (T.gmshAtoJ >= 3) & (T.gmshAtoK >= 3))[0]   # local volume nodes
+
+
+def _placeholder(m, gas):
+    npn, ne = m.npoin, m.nelem
+    one, zero = np.ones(npn), np.zeros(npn)
+    z3, ze = np.zeros((npn, 3), order="F"), np.zeros((ne, 64), order="F")
+    return dict(rho=one, u=z3, q=z3, pr=one, E=one, Tem=one, e_int=one, eta=zero, csound=one, machno=zero,
+                mu_fluid=zero, mu_factor=np.full(npn, gas.mu_factor), mu_e=ze, mu_sgs=ze, mue_l=ze)
+
+
+def _fourier(m, seed, amp=1e-2, kmax=8.0):
+    """cases.fourier_velocity_perturbation with the separable sums as matrix
+    products (npoin, 3)."""
+    rng = np.random.default_rng(seed)
+    km = int(np.floor(kmax))
+    ks = np.arange(-km, km + 1)
+    K1, K2, K3 = np.meshgrid(ks, ks, ks, indexing="ij")
+    mask = (K1 ** 2 + K2 ** 2 + K3 ** 2 <= kmax ** 2) & ~((K1 == 0) & (K2 == 0) & (K3 == 0))
+    r = hex64_tables().gll_roots
+    axes = []
+    for nel, Ld in zip(m.n, m.L):
+        idx = np.arange(3 * nel + 1)
+        h = Ld / nel
+        e = np.minimum(idx // 3, nel - 1)
+        xs = e * h + (1.0 + r[idx - 3 * e]) * (0.5 * h)
+        axes.append(np.exp(1j * np.outer(ks, 2 * np.pi * xs / Ld)))
+    nk, lat = len(ks), m.node_lattice
+    out = np.zeros((m.npoin, 3))
+    for c in range(3):
+        a = (rng.standard_normal(K1.shape) + 1j * rng.standard_normal(K1.shape)) * amp
+        a = np.where(mask, a, 0.0)
+        M = [ax.shape[1] for ax in axes]
+        t1 = a.reshape(nk, nk * nk).T @ axes[0]                                  # (b c, I)
+        t2 = t1.reshape(nk, nk * M[0]).T @ axes[1]                               # (c I, J)
+        f = (t2.reshape(nk, M[0] * M[1]).T @ axes[2]).real.reshape(M[0], M[1], M[2])
+        out[:, c] = f[lat[:, 0], lat[:, 1], lat[:, 2]]
+    return out
+
+
+def _device_state(sol, m, gas, modes_seed):
+    """Level-2 state of tgv_initial_state written into the solver's arrays."""
+    npn = m.npoin
+    xyz = sol.coord.view(3, npn)
+    x, y, z = (xyz[d].float() for d in range(3))          # coordinates read as real(4) (TGVSolver.f90:121-123)
+    V0, L = float(np.float32(gas.V0)), float(np.float32(gas.L))
+    u = torch.zeros(3, npn, dtype=torch.float64, device="cuda")
+    u[0] = (V0 * torch.sin(x / L) * torch.cos(y / L) * torch.cos(z / L)).double()
+    u[1] = (-V0 * torch.cos(x / L) * torch.sin(y / L) * torch.cos(z / L)).double()
+    xd, yd, zd = x.double(), y.double(), z.double()
+    Lf = float(gas.L)
+    pr = gas.po + ((gas.V0 * gas.V0) / 16.0) * (torch.cos(2.0 * xd / Lf) + torch.cos(2.0 * yd / Lf)) * (
+        torch.cos(2.0 * zd / Lf) + 2.0)
+    if modes_seed is not None:
+        u += torch.from_numpy(np.ascontiguousarray(_fourier(m, modes_seed).T)).cuda()
+    rho = pr / gas.Rgas / gas.to
+    gm = gas.gamma_gas
+    e_int = pr / (rho * (gm - 1.0))
+    Tem = pr / (rho * gas.Rgas)
+    E = rho * (0.5 * (u * u).sum(0) + e_int)
+    cs = torch.sqrt(gm * pr / rho)
+    eta = (rho / (gm - 1.0)) * torch.log(pr / rho ** gm)
+    mu = sol.mu_factor * 0.000001458 * Tem ** 1.5 / (Tem + 110.4)
+    sol.level("rho").copy_(rho)
+    sol.level("u").copy_(u.reshape(-1))
+    sol.level("q").copy_((rho[None, :] * u).reshape(-1))
+    sol.level("pr").copy_(pr)
+    sol.level("E").copy_(E)
+    sol.level("Tem").copy_(Tem)
+    sol.level("e_int").copy_(e_int)
+    sol.level("eta").copy_(eta)
+    sol.csound.copy_(cs)
+    sol.mu_fluid.copy_(mu)
+    torch.cuda.synchronize()
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def big(request, gpu):
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    kw = CASES[request.param]
+    print(f"[fullsize] {request.param}: building the mesh", flush=True)
+    m = pmesh.periodic_box(kw["n"], curved_amp=kw["curved_amp"], perm_seed=kw["perm_seed"])
+    gas = cases.GasParams(Re=kw["Re"], Mach=kw["Mach"])
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    print(f"[fullsize] {request.param}: {m.nelem} elements, {m.npoin_w} working nodes; solver set-up", flush=True)
+    sol = PeriodicSolver(m, gas, _placeholder(m, gas))
+    _device_state(sol, m, gas, kw["modes_seed"])
+    print(f"[fullsize] {request.param}: state ready", flush=True)
+    yield request.param, m, gas, sol
+    sol.close()
+    del sol
+    torch.cuda.empty_cache()
+
+
+def test_rhs_of_sampled_elements_matches_oracle(big):
+    name, m, gas, sol = big
+    ne, npn = m.nelem, m.npoin
+    ts, saved = _snapshot(sol)
+    if CASES[name]["modes_seed"] is None:
+        _device_state(sol, m, gas, 3)
+    gen = torch.Generator(device="cuda").manual_seed(17)
+    mu_e = torch.rand(ne * 64, dtype=torch.float64, device="cuda", generator=gen) * 2e-3
+    z = lambda k: torch.zeros(k, dtype=torch.float64, device="cuda")
+    Rm, Rq, Re, Rt = z(npn), z(3 * npn), z(npn), z(npn)
+    sgs = z(ne * 64)
+    _lib.call("sod_full_rhs", ne, npn, m.npoin_w, ptr(sol.connec), ptr(sol.lpoin_w), ptr(sol.He), ptr(sol.gpvol),
+              ptr(sol.Ml), ptr(sol.level("rho")), ptr(sol.level("q")), ptr(sol.level("E")), ptr(sol.mu_factor),
+              ptr(mu_e), ptr(sgs), gas.Rgas, gas.gamma_gas, gas.Cp, gas.Prt, ptr(Rm), ptr(Rq), ptr(Re), ptr(Rt))
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(5)
+    elems = rng.choice(ne, size=48, replace=False)
+    got = {k: [] for k in ("Rm", "Rq", "Re", "Rt")}
+    ref = {k: [] for k in got}
+    c1 = np.asfortranarray(np.arange(1, 65, dtype=np.int32).reshape(1, 64))
+    rho2, q2, E2 = sol.level("rho"), sol.level("q").view(3, npn), sol.level("E")
+    for e in elems:
+        g = torch.from_numpy(np.asarray(m.connec[e], dtype=np.int64) - 1).cuda()
+        st = {"rho": rho2[g].cpu().numpy(), "q": np.asfortranarray(q2[:, g].cpu().numpy().T), "E": E2[g].cpu().numpy()}
+        mf = sol.mu_factor[g].cpu().numpy()
+        He = np.asfortranarray(sol.He[576 * e:576 * (e + 1)].cpu().numpy().reshape(3, 3, 64, 1, order="F"))
+        gv = np.asfortranarray(sol.gpvol[64 * e:64 * (e + 1)].cpu().numpy().reshape(1, 64, 1, order="F"))
+        mue = np.asfortranarray(mu_e[e + ne * torch.arange(64, device="cuda")].cpu().numpy().reshape(1, 64))
+        f = stage_fields(st, gas.gamma_gas, gas.Rgas)
+        mu = oracle.sutherland_viscosity(f["T"], mf)
+        cm, cq, ce = oracle.full_convec_ijk(c1, He, gv, T, f["u"], st["q"], st["rho"], f["p"], f["h"])
+        dm, dq, de = oracle.full_diffusion_ijk(c1, He, gv, T, gas.Cp, gas.Prt, st["rho"], st["rho"], f["u"], f["T"],
+                                               mu, mue, np.zeros((1, 64), order="F"))
+        rt = oracle.generic_scalar_convec_ijk(c1, He, gv, T, f["feta"], f["eta"], f["u"])
+        gi = g[torch.from_numpy(INTERIOR).cuda()]
+        got["Rm"].append(Rm[gi].cpu().numpy())
+        got["Rq"].append(Rq.view(3, npn)[:, gi].cpu().numpy().T)
+        got["Re"].append(Re[gi].cpu().numpy())
+        got["Rt"].append(Rt[gi].cpu().numpy())
+        ref["Rm"].append((cm + dm)[INTERIOR])
+        ref["Rq"].append((cq + dq)[INTERIOR])
+        ref["Re"].append((ce + de)[INTERIOR])
+        ref["Rt"].append(rt[INTERIOR])
+    errs = {}
+    for k in got:
+        errs[k] = rel_l2(np.concatenate(got[k]), np.concatenate(ref[k]))
+        print(f"  {name} sampled RHS {k} rel_l2={errs[k]:.3e}", flush=True)
+    for k, v in ts.items():
+        v.copy_(saved[k])
+    torch.cuda.synchronize()
+    for k, err in errs.items():
+        assert err < 1e-12, (k, err)
+
+
+def _sums(sol, m):
+    w = sol.lpoin_w.long() - 1
+    Ml = sol.Ml[w]
+    npn = m.npoin
+    mass = float((Ml * sol.level("rho")[w]).sum())
+    mom = (Ml[None, :] * sol.level("q").view(3, npn)[:, w]).sum(1).cpu().numpy()
+    return mass, mom
+
+
+def _snapshot(sol):
+    ts = {k: v for k, v in vars(sol).items() if isinstance(v, torch.Tensor)}
+    ts.update({"lv." + k: v for k, v in sol.lv.items()})
+    return ts, {k: v.clone() for k, v in ts.items()}
+
+
+def test_conservation_and_reproducibility(big):
+    name, m, gas, sol = big
+    ts, saved = _snapshot(sol)
+    dt = sol.adapt_dt()
+    m0, q0 = _sums(sol, m)
+    print(f"  {name}: dt={dt:.4e} mass={m0:.6e} momentum={q0}", flush=True)
+    finals = []
+    for rep in range(2):
+        for k, v in ts.items():
+            v.copy_(saved[k])
+        torch.cuda.synchronize()
+        for _ in range(3):
+            sol.rk_4_main(dt)
+        torch.cuda.synchronize()
+        finals.append({k: sol.level(k).clone() for k in ("rho", "q", "E")} | {"mu_e": sol.mu_e.clone()})
+    for k in finals[0]:
+        assert torch.equal(finals[0][k], finals[1][k]), k
+    assert float(sol.mu_e.abs().max()) > 0.0
+    _, q1 = _sums(sol, m)
+    np.testing.assert_allclose(q1, q0, atol=1e-11 * m0)
+    # mass, with the entropy viscosity off
+    p0 = numerical_params(Rgas_inf=gas.Rgas)
+    try:
+        set_numerical_params(numerical_params(Rgas_inf=gas.Rgas, cglob=0.0))
+        for k, v in ts.items():
+            v.copy_(saved[k])
+        torch.cuda.synchronize()
+        for _ in range(3):
+            sol.rk_4_main(dt)
+        torch.cuda.synchronize()
+        m1, _ = _sums(sol, m)
+        print(f"  {name}: mass drift with cglob=0: {abs(m1 / m0 - 1):.2e}", flush=True)
+        assert abs(m1 / m0 - 1) < 1e-12
+    finally:
+        set_numerical_params(p0)
+    for k, v in ts.items():
+        v.copy_(saved[k])
+    torch.cuda.synchronize()
