@@ -14,7 +14,10 @@ with the mesh:
 - momentum is conserved over RK4 steps (split-form convection and weak-form
   viscous terms are conservative), and mass too with the entropy viscosity
   off (cglob = 0), as the oracle shows on small meshes;
-- steps from one saved state are bit-reproducible.
+- steps from one saved state are bit-reproducible;
+- the differentiation case at its full size (24^3, 200 RK4 steps): the RHS
+  adjoint against central differences with the entropy viscosity frozen, and
+  the checkpointed gradient of the full step against one 200-step segment.
 
 The initial states follow sod2d_refmap_amd.cases.tgv_initial_state; the
 element-wise part is evaluated on the device from the solver's coordinates
@@ -237,3 +240,87 @@ def test_conservation_and_reproducibility(big):
     for k, v in ts.items():
         v.copy_(saved[k])
     torch.cuda.synchronize()
+
+
+# ------------------------------------------------ adjoint at full size
+def _adjoint_case():
+    """The differentiation case at its full size: Taylor-Green on 24^3
+    elements, Re 1600, Mach 0.1, 200 RK4 steps at the initial CFL 0.5 step."""
+    from _util import mesh_geometry
+    from sod2d_refmap_amd.autograd import EVContext, RHSContext
+    m = pmesh.periodic_box(24)
+    g = mesh_geometry(m)
+    gas = cases.GasParams(Re=1600.0, Mach=0.1)
+    s = cases.tgv_initial_state(m, gas)
+    cu = lambda a: torch.tensor(np.asarray(a), dtype=torch.float64, device="cuda")
+    rho0, p0, u0 = cu(s["rho"]), cu(s["pr"]), cu(s["u"])
+    g1 = gas.gamma_gas - 1.0
+
+    def initial(u):   # velocity varied at fixed density and pressure
+        return rho0 * 0 + rho0, rho0[:, None] * u, p0 / g1 + 0.5 * rho0 * (u * u).sum(-1)
+
+    dt = oracle.adapt_dt_cfl(m.connec, g["helem"], s["u"], s["csound"], 0.5, 0.5, s["mu_fluid"],
+                             np.zeros((m.nelem, 64), order="F"), s["rho"])
+
+    def contexts():
+        rc = RHSContext.from_mesh(m, g["He"], g["gpvol"], g["Ml"], s["mu_factor"], gas)
+        return rc, EVContext.create(rc, m, g["helem"], g["helem_l"], T)
+    return m, gas, u0, initial, dt, contexts
+
+
+def test_adjoint_200_steps_frozen_viscosity_central_differences(gpu):
+    """d(kinetic energy after 200 steps)/d(initial velocity) through the
+    hand-written RHS adjoint, with the entropy viscosity held fixed (a smooth
+    map), against central differences (step 1e-6) along 8 seeded random
+    directions of unit RMS per degree of freedom.  Directions of unit L2 norm
+    give directional derivatives of ~1e-7 against an objective of ~0.1, and
+    the FP64 rounding of the objective then limits the difference quotient
+    to ~1e-5."""
+    from sod2d_refmap_amd.autograd import kinetic_energy, rk4_step
+    m, gas, u0, initial, dt, contexts = _adjoint_case()
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    rc, _ = contexts()
+    mu_e = torch.tensor(np.random.default_rng(7).uniform(0, 2e-3, (m.nelem, 64)), dtype=torch.float64,
+                        device="cuda")
+
+    def run(u):
+        r, q, E = initial(u)
+        for _ in range(200):
+            r, q, E, _ = rk4_step(rc, r, q, E, mu_e, dt)
+        return kinetic_energy(rc, r, q)
+
+    u = u0.clone().requires_grad_(True)
+    (gu,) = torch.autograd.grad(run(u), u)
+    rng = np.random.default_rng(3)
+    h = 1e-6
+    for it in range(8):
+        v = torch.tensor(rng.standard_normal((m.npoin, 3)), dtype=torch.float64, device="cuda")
+        v = v * (v.numel() ** 0.5 / torch.linalg.norm(v))
+        with torch.no_grad():
+            fd = (float(run(u0 + h * v)) - float(run(u0 - h * v))) / (2 * h)
+        an = float((gu * v).sum())
+        print(f"  frozen-viscosity adjoint, direction {it}: fd={fd:.12e} adjoint={an:.12e}", flush=True)
+        assert abs(fd - an) <= 1e-6 * abs(an), (it, fd, an)
+
+
+def test_adjoint_200_steps_checkpointed_equals_one_segment(gpu):
+    """The full differentiable step (RK4 stages, final EOS and the entropy
+    viscosity, its limiters as subgradients): the gradient with a checkpoint
+    every 20 steps equals the one taken through a single 200-step segment."""
+    from sod2d_refmap_amd.autograd import run_checkpointed
+    m, gas, u0, initial, dt, contexts = _adjoint_case()
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas, flag_normalise_entropy=0))
+    try:
+        _, ec = contexts()
+        mu0 = torch.zeros((m.nelem, 64), dtype=torch.float64, device="cuda")
+        grads = []
+        for every in (20, 200):
+            u = u0.clone().requires_grad_(True)
+            J = run_checkpointed(ec, *initial(u), mu0, [dt] * 200, every=every)
+            (gu,) = torch.autograd.grad(J, u)
+            grads.append(gu)
+            print(f"  checkpoint every {every}: J={float(J.detach()):.15e} |grad|={float(gu.norm()):.6e}", flush=True)
+        err = float((grads[0] - grads[1]).norm() / grads[1].norm())
+        assert err <= 1e-13, err
+    finally:
+        set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
