@@ -324,3 +324,33 @@ def test_adjoint_200_steps_checkpointed_equals_one_segment(gpu):
         assert err <= 1e-13, err
     finally:
         set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+
+
+# ------------------------------------------------ ensemble at full size
+def test_ensemble_of_48_matches_members_stepped_one_by_one(gpu):
+    """The batching case at its full size: 48 seeded Taylor-Green boxes of
+    8^3 to 32^3 elements (Re, Mach and Fourier modes seeded), 3 steps of the
+    batched device loop against the same steps member by member through the
+    plain entry points (whose parity with the oracle the small-mesh tests
+    show): relative L2 <= 1e-12 per member and field, dt and time equal to
+    1e-12."""
+    from sod2d_refmap_amd.ensemble import Ensemble, ensemble_members, sequential_solvers, sequential_step
+    members = ensemble_members(48)
+    print(f"[fullsize] ensemble: {sum(mm.nelem for mm, _, _ in members)} elements in 48 members", flush=True)
+    E = Ensemble(members)
+    sols = sequential_solvers(members)
+    E.run(3, keep_levels=False)
+    for _ in range(3):
+        sequential_step(sols)
+    worst = 0.0
+    for i, sol in enumerate(sols):
+        a, b = E.member_state(i), sol.state()
+        for k in ("rho", "q", "E", "u", "pr", "eta", "mu_fluid", "mu_e"):
+            assert np.isfinite(a[k]).all(), (i, k)
+            err = rel_l2(a[k], b[k])
+            worst = max(worst, err)
+            assert err < 1e-12, (i, k, err)
+        assert abs(E.dt[i] / sol.dt - 1) < 1e-12 and abs(E.time[i] / sol.time - 1) < 1e-12, i
+        sol.close()
+    print(f"  ensemble of 48: largest relative L2 difference {worst:.2e}", flush=True)
+    E.close()
