@@ -69,10 +69,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
     __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
-    // on XCD b % 8); give XCD x the contiguous range of element blocks
-    // [x*q + min(x, r8), ...) so that neighbouring elements, which share
-    // nodes, meet in the same L2 for their gathers and atomics.
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    // on XCD b % 8).  Multi-element workgroups give XCD x the contiguous
+    // range of element blocks [x*q + min(x, r8), ...) so that neighbouring
+    // elements, which share nodes, meet in the same L2; with one element per
+    // workgroup the plain order measured 1.6% faster at 64^3.
+    const int lb = NW == 1 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     const int e = lb * NW + w;
     if (e >= nelem) return;   // uniform per wavefront; only wave-level sync below
     const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
