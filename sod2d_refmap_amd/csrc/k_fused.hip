@@ -589,7 +589,7 @@ __global__ __launch_bounds__(256) void k_smart_visc_lane(
         s_w[threadIdx.x] = c_tab.wgp[a];
     }
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int e0 = xcd_block(blockIdx.x, gridDim.x) * 16;
+    const int e0 = blockIdx.x * 16;   // plain dispatch order (2% faster than the XCD remapping)
     const size_t np = (size_t)npoin, ne = (size_t)nelem;
     // helem_l(nelem,64) of the block's elements, read contiguously into s_me
     for (int idx = threadIdx.x; idx < 1024; idx += 256) {
