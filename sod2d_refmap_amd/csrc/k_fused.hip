@@ -335,6 +335,12 @@ struct StepState {
 
 constexpr int RED_BLOCKS = 1024;
 
+// Grid of the node update of stages 1-3: one thread per working node (no
+// grid-stride loop), 6% faster than 1024 or 4096 resident blocks at 64^3.
+// The final stage keeps RED_BLOCKS blocks: it leaves one partial sum of eta
+// per block, and one block sums them.
+inline int node_grid(int npoin_w) { return std::max(1, grid_for(npoin_w, BLK)); }
+
 // Stage s in 0..3 after its element kernel: lumped solve, accumulation,
 // next stage state (s < 3) or final update + EOS (s == 3).
 // With BATCHED the nodes belong to independent instances (a flattened
@@ -1133,8 +1139,9 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
             launch_stage_rhs(P.cfg, has_sgs, s, nelem, P, in, (long)np, sp);
         }
         SOD_PROF("stage_update");
-        launch_update(stage == 3, B != nullptr, kStageCfgs[P.cfg].atomic, dim3(ugrid), s, npoin, npoin_w, a.lpoin_w,
-                      stage, d_dt, a.gamma_gas, a.Rgas, st, B ? B->node_inst : nullptr);
+        const dim3 grid(stage == 3 ? ugrid : node_grid(npoin_w));
+        launch_update(stage == 3, B != nullptr, kStageCfgs[P.cfg].atomic, grid, s, npoin, npoin_w, a.lpoin_w, stage,
+                      d_dt, a.gamma_gas, a.Rgas, st, B ? B->node_inst : nullptr);
     }
     if (P.nslave > 0)
         hipLaunchKernelGGL(k_slave_sutherland, dim3(grid_for(P.nslave, BLK)), dim3(BLK), 0, s, P.nslave,
@@ -1204,7 +1211,6 @@ static int choose_stage_cfg(Prepared& P, const FusedArgs& a, bool has_sgs, const
     const StepState st = step_state(P, a);
     const sod_numerical_params& prm = rt().params;
     const StageParams sp = make_stage_params(a.gamma_gas, a.Rgas, a.Cp, a.Prt, prm.c_rho, prm.c_ener);
-    const int ugrid = std::min(RED_BLOCKS, std::max(1, grid_for(a.npoin_w, BLK)));
     const double* const in[6] = {a.rho, a.q, a.q + np, a.q + 2 * np, a.E, a.mu_factor};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     SOD_HIP(hipEventCreate(&e0));
@@ -1217,7 +1223,8 @@ static int choose_stage_cfg(Prepared& P, const FusedArgs& a, bool has_sgs, const
         for (int rep = 0; rep < 4 && err == hipSuccess; ++rep) {
             err = hipEventRecord(e0, s);
             if (a.nelem > 0) launch_stage_rhs(c, has_sgs, s, a.nelem, P, in, (long)np, sp);
-            launch_update(false, B != nullptr, kStageCfgs[c].atomic, dim3(ugrid), s, a.npoin, a.npoin_w, a.lpoin_w,
+            launch_update(false, B != nullptr, kStageCfgs[c].atomic, dim3(node_grid(a.npoin_w)), s, a.npoin,
+                          a.npoin_w, a.lpoin_w,
                           0, d_dt, a.gamma_gas, a.Rgas, st, B ? B->node_inst : nullptr);
             if (err == hipSuccess) err = hipGetLastError();
             if (err == hipSuccess) err = hipEventRecord(e1, s);
