@@ -799,7 +799,7 @@ extern "C" int sod_full_rhs_vjp(int nelem, int npoin, int npoin_w, const int* co
         hipLaunchKernelGGL(k_lane_to_ref_tiled, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, gml, dgmu);
     }
     if (npoin > 0)
-        hipLaunchKernelGGL(k_assemble_grad, dim3(std::min(2048, grid_for(npoin, 256))), dim3(256), 0, s, npoin,
+        hipLaunchKernelGGL(k_assemble_grad, dim3(std::max(1, grid_for(npoin, 256))), dim3(256), 0, s, npoin,
                            lm.inv_off, lm.inv_pos, lm.relem, dgr, dgq, dgE);
     SOD_HIP(hipGetLastError());
     return finish_call();

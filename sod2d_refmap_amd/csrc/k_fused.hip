@@ -1309,7 +1309,7 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
         launch_stage_rhs(0, has_sgs != 0, s, nelem, Pm, in, (long)np, sp);
     }
     if (npoin > 0)
-        hipLaunchKernelGGL(k_assemble_nodes, dim3(std::min(2048, grid_for(npoin, 256))), dim3(256), 0, s, npoin,
+        hipLaunchKernelGGL(k_assemble_nodes, dim3(std::max(1, grid_for(npoin, 256))), dim3(256), 0, s, npoin,
                            Pm.inv_off, Pm.inv_pos, Pm.relem, Rmass, Rmom, Rener, Reta);
     SOD_HIP(hipGetLastError());
     return SOD_OK;
