@@ -677,37 +677,29 @@ __global__ __launch_bounds__(256) void k_smart_visc_lane(
 
 // Position-weighted fingerprint of sampled mesh data (connec, He, gpvol,
 // lpoin_w, Ml) so that a prepared mesh is rebuilt when a caller reuses the
-// same addresses for different contents.
+// same addresses for different contents.  Workgroup r (16 of them) samples
+// round r and leaves its 4 sums in out[4r .. 4r+3]; the host adds the rounds
+// in order (fingerprint()).
 __global__ __launch_bounds__(256) void k_fingerprint(int nelem, int npoin_w, const int* __restrict__ connec,
                                                      const double* __restrict__ He,
                                                      const double* __restrict__ gpvol,
                                                      const int* __restrict__ lpoin_w,
                                                      const double* __restrict__ Ml, double* __restrict__ out) {
     __shared__ double s[4][256];
-    const int t = threadIdx.x;
-    double a = 0, b = 0, c = 0, d = 0;
-    // unrolled so that the 16 rounds' loads are issued together (the sums
-    // keep their order)
-#pragma unroll
-    for (int smp = 0; smp < 16; ++smp) {
-        const int e = (int)(((long)nelem * smp + (t >> 4)) / 16) % nelem;
-        const int nd = t & 63;
-        const double wgt = 1.0 + 0.001 * (t + 256 * smp);
-        a += wgt * connec[(size_t)e + (size_t)nelem * nd];
-        b += wgt * He[576 * (size_t)e + (t % 576)];
-        c += wgt * gpvol[64 * (size_t)e + nd];
-        const int w = (int)(((long)npoin_w * (t + 256 * smp)) / 4096) % npoin_w;
-        d += wgt * lpoin_w[w] + wgt * Ml[lpoin_w[w] - 1];
-    }
-    s[0][t] = a;
-    s[1][t] = b;
-    s[2][t] = c;
-    s[3][t] = d;
+    const int t = threadIdx.x, smp = blockIdx.x;
+    const int e = (int)(((long)nelem * smp + (t >> 4)) / 16) % nelem;
+    const int nd = t & 63;
+    const double wgt = 1.0 + 0.001 * (t + 256 * smp);
+    const int w = (int)(((long)npoin_w * (t + 256 * smp)) / 4096) % npoin_w;
+    s[0][t] = wgt * connec[(size_t)e + (size_t)nelem * nd];
+    s[1][t] = wgt * He[576 * (size_t)e + (t % 576)];
+    s[2][t] = wgt * gpvol[64 * (size_t)e + nd];
+    s[3][t] = wgt * lpoin_w[w] + wgt * Ml[lpoin_w[w] - 1];
     __syncthreads();
     if (t < 4) {
         double v = 0;
         for (int q = 0; q < 256; ++q) v += s[t][q];
-        out[t] = v;
+        out[4 * smp + t] = v;
     }
 }
 
@@ -802,6 +794,25 @@ __global__ __launch_bounds__(256) void k_check_conn(long n, const int* __restric
     if (b) *bad = 1;
 }
 
+constexpr int kFpRounds = 16;
+
+// The fingerprint of the mesh arrays (k_fingerprint): the rounds' sums added
+// in round order on the host.  buf holds 4 * kFpRounds doubles.
+static int fingerprint(int nelem, int npoin_w, const int* connec, const double* He, const double* gpvol,
+                       const int* lpoin_w, const double* Ml, double* buf, double fp[4]) {
+    hipLaunchKernelGGL(k_fingerprint, dim3(kFpRounds), dim3(256), 0, rt().stream, nelem, npoin_w, connec, He, gpvol,
+                       lpoin_w, Ml, buf);
+    double h[4 * kFpRounds];
+    SOD_HIP(hipMemcpyAsync(h, buf, sizeof(h), hipMemcpyDeviceToHost, rt().stream));
+    SOD_HIP(hipStreamSynchronize(rt().stream));
+    for (int t = 0; t < 4; ++t) {
+        double v = 0.0;
+        for (int r = 0; r < kFpRounds; ++r) v += h[4 * r + t];
+        fp[t] = v;
+    }
+    return SOD_OK;
+}
+
 // The inverse connectivity holds element-node positions e*64 + lane as int32.
 static int check_mesh_size(int nelem) {
     if ((long long)nelem * 64 > (long long)INT_MAX)
@@ -818,10 +829,7 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
     for (int t = 0; t < 5 && same; ++t) same = P.key[t] == key[t];
     double fp[4] = {0, 0, 0, 0};
     if (same && nelem > 0 && npoin_w > 0) {
-        hipLaunchKernelGGL(k_fingerprint, dim3(1), dim3(256), 0, rt().stream, nelem, npoin_w, connec, He, gpvol,
-                           lpoin_w, Ml, P.fpbuf);
-        SOD_HIP(hipMemcpyAsync(fp, P.fpbuf, sizeof(fp), hipMemcpyDeviceToHost, rt().stream));
-        SOD_HIP(hipStreamSynchronize(rt().stream));
+        SOD_TRY(fingerprint(nelem, npoin_w, connec, He, gpvol, lpoin_w, Ml, P.fpbuf, fp));
         if (std::memcmp(fp, P.fp, sizeof(fp)) == 0) return SOD_OK;
     }
     SOD_HIP(hipStreamSynchronize(rt().stream));
@@ -834,7 +842,7 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
     SOD_HIP(hipMalloc(&P.musgs, sizeof(double) * ne * 64 + 8));
     SOD_HIP(hipMalloc(&P.flag, sizeof(int) * 4));
     SOD_HIP(hipMalloc(&P.dscal, sizeof(double) * 8));
-    SOD_HIP(hipMalloc(&P.fpbuf, sizeof(double) * 4));
+    SOD_HIP(hipMalloc(&P.fpbuf, sizeof(double) * 4 * kFpRounds));
     P.work_doubles = np * 11 + 2 * RED_BLOCKS + 64;
     SOD_HIP(hipMalloc(&P.work, sizeof(double) * P.work_doubles));
     SOD_HIP(hipMemsetAsync(P.work, 0, sizeof(double) * P.work_doubles, rt().stream));
@@ -886,12 +894,7 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
     P.nslave = (int)sl.size();
     SOD_HIP(hipMalloc(&P.slaves, sizeof(int) * (sl.size() + 1)));
     if (!sl.empty()) SOD_HIP(hipMemcpy(P.slaves, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice));
-    if (nelem > 0 && npoin_w > 0) {
-        hipLaunchKernelGGL(k_fingerprint, dim3(1), dim3(256), 0, rt().stream, nelem, npoin_w, connec, He, gpvol,
-                           lpoin_w, Ml, P.fpbuf);
-        SOD_HIP(hipMemcpyAsync(P.fp, P.fpbuf, sizeof(P.fp), hipMemcpyDeviceToHost, rt().stream));
-        SOD_HIP(hipStreamSynchronize(rt().stream));
-    }
+    if (nelem > 0 && npoin_w > 0) SOD_TRY(fingerprint(nelem, npoin_w, connec, He, gpvol, lpoin_w, Ml, P.fpbuf, P.fp));
     SOD_HIP(hipGetLastError());
     for (int t = 0; t < 5; ++t) P.key[t] = key[t];
     P.nelem = nelem;
