@@ -135,7 +135,7 @@ def test_time_loop_across_configurations(gpu, default_after):
     B.close()
 
 
-def test_ensemble_with_atomic_scatter(gpu, default_after):
+def test_ensemble_with_atomic_scatter_and_tuned(gpu, default_after):
     from sod2d_refmap_amd.ensemble import Ensemble
     members = []
     for n, amp, perm, Re, Ma, seed in [(2, 0.0, None, 800.0, 0.1, 1), (3, 0.05, 9, 1600.0, 0.1, 2)]:
@@ -149,10 +149,19 @@ def test_ensemble_with_atomic_scatter(gpu, default_after):
     E.run(3)
     c = E.stage_config()
     assert (c["elems_per_workgroup"], c["atomic_scatter"]) == (4, True)
-    for i in range(len(members)):
-        got, ref = E.member_state(i), D.member_state(i)
-        for k in KEYS:
-            err = rel_l2(got[k], ref[k])
-            assert err < 1e-12, (i, k, err)
+    # the tuner on a batched loop (per-instance dt in its trial stages)
+    _lib.set_autotune(2)
+    F = Ensemble(members)
+    F.run(3)
+    cf = F.stage_config()
+    print(f"[stage config] tuned ensemble: {cf}", flush=True)
+    assert cf["mode"] == 2 and cf["stage_ms"] > 0.0
+    for X in (E, F):
+        for i in range(len(members)):
+            got, ref = X.member_state(i), D.member_state(i)
+            for k in KEYS:
+                err = rel_l2(got[k], ref[k])
+                assert err < 1e-12, (i, k, err)
     D.close()
     E.close()
+    F.close()
