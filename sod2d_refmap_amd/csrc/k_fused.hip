@@ -541,7 +541,8 @@ __global__ void k_inst_norms(int ninst, const double* __restrict__ stats, double
 // Tiled transpose of up to two element-node arrays (reference layout
 // (nelem,64), element fastest) into lane layout [e][lane]; 16 elements per
 // block so that both the reads and the writes are contiguous.  `nonzero` is
-// set when the second array holds a non-zero value.
+// set when the second array holds a non-zero value; with dstB null the second
+// array is only scanned for that flag.
 __global__ __launch_bounds__(256) void k_to_lane_tiled(int nelem, const double* __restrict__ srcA,
                                                        double* __restrict__ dstA, const double* __restrict__ srcB,
                                                        double* __restrict__ dstB, int* __restrict__ nonzero) {
@@ -557,7 +558,7 @@ __global__ __launch_bounds__(256) void k_to_lane_tiled(int nelem, const double* 
             sA[el][a] = srcA[(size_t)e + ne * a];
             if (srcB) {
                 const double v = srcB[(size_t)e + ne * a];
-                sB[el][a] = v;
+                if (dstB) sB[el][a] = v;
                 nz |= v != 0.0;
             }
         }
@@ -567,7 +568,7 @@ __global__ __launch_bounds__(256) void k_to_lane_tiled(int nelem, const double* 
         const int el = idx >> 6, l = idx & 63, e = e0 + el;
         if (e < nelem) {
             dstA[(size_t)e * 64 + l] = sA[el][s_l2n[l]];
-            if (srcB) dstB[(size_t)e * 64 + l] = sB[el][s_l2n[l]];
+            if (dstB) dstB[(size_t)e * 64 + l] = sB[el][s_l2n[l]];
         }
     }
     if (nz && nonzero) *nonzero = 1;
@@ -953,17 +954,23 @@ static int prepare_cached(Prepared** out, int nelem, int npoin, int npoin_w, con
     return SOD_OK;
 }
 
-// mu_e (and mu_sgs) from the caller's layout into P's lane layout; returns
-// whether mu_sgs holds a non-zero value (host synchronisation).
+// mu_e from the caller's layout into P's lane layout, and mu_sgs too when it
+// holds a non-zero value (returned in has_sgs; host synchronisation).  The
+// kernels read P.musgs only in that case, so an all-zero mu_sgs, the usual
+// one, costs a read and no write.
 static int load_viscosities(Prepared& P, int nelem, const double* mu_e, const double* mu_sgs, int* has_sgs) {
     hipStream_t s = rt().stream;
     *has_sgs = 0;
     if (nelem <= 0) return SOD_OK;
+    const dim3 grid(grid_for(nelem, 16));
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, P.flag);
-    hipLaunchKernelGGL(k_to_lane_tiled, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, mu_e, P.mue, mu_sgs,
-                       P.musgs, P.flag);
+    hipLaunchKernelGGL(k_to_lane_tiled, grid, dim3(256), 0, s, nelem, mu_e, P.mue, mu_sgs, (double*)nullptr,
+                       P.flag);
     SOD_HIP(hipMemcpyAsync(has_sgs, P.flag, sizeof(int), hipMemcpyDeviceToHost, s));
     SOD_HIP(hipStreamSynchronize(s));
+    if (*has_sgs)
+        hipLaunchKernelGGL(k_to_lane_tiled, grid, dim3(256), 0, s, nelem, mu_sgs, P.musgs, (const double*)nullptr,
+                           (double*)nullptr, (int*)nullptr);
     return SOD_OK;
 }
 

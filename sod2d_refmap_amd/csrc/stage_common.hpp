@@ -89,7 +89,8 @@ enum { R_MASS, R_MOM0, R_MOM1, R_MOM2, R_ENER, R_ETA, NRES };
 
 // Lane-layout view of the mesh prepared for the drop-in entry points
 // (k_fused.hip): conn[e][lane] (0-based), He[e][d+3c][lane], gpv/mue/musgs
-// [e][lane]; mue/musgs hold the caller's mu_e / mu_sgs of the last call.
+// [e][lane]; mue holds the caller's mu_e of the last call, musgs its mu_sgs
+// when has_sgs (and is not read otherwise).
 struct LaneMesh {
     const int* conn;
     const double *He, *gpv, *mue, *musgs;
