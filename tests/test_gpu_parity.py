@@ -23,6 +23,9 @@ TOL1 = 1e-12
 TOL10 = 1e-10
 
 MESHES = {
+    # one periodic element: 27 working nodes, up to 8 positions of one node
+    # inside the same element
+    "single1": dict(n=1),
     "straight2": dict(n=2),
     "straight4": dict(n=4),
     "curved3_perm": dict(n=3, curved_amp=0.05, perm_seed=11),
