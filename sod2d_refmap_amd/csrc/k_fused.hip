@@ -864,7 +864,7 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
         P.release();
         return fail(SOD_ERR_ARG, "connec holds an id outside 1..npoin");
     }
-    SOD_HIP(hipMemcpy(hw.data(), lpoin_w, sizeof(int) * npoin_w, hipMemcpyDeviceToHost));
+    if (npoin_w > 0) SOD_HIP(hipMemcpy(hw.data(), lpoin_w, sizeof(int) * npoin_w, hipMemcpyDeviceToHost));
     {
         // node -> (element, lane) positions of the lane-layout connectivity,
         // by a counting sort that keeps the positions ascending

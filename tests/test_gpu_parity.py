@@ -328,3 +328,27 @@ def test_mesh_size_limit_is_rejected(gpu):
     with pytest.raises(_lib.Sod2dError, match="at most 33554431 elements"):
         _lib.call("sod_full_rhs", ne, 8, 8, zi.data_ptr(), zi.data_ptr(), *[z.data_ptr()] * 9, 287.0, 1.4, 1004.0,
                   0.71, *[z.data_ptr()] * 4)
+
+
+def test_empty_mesh(gpu):
+    """A part of a mesh with no elements and no nodes.  The reference's loops
+    then run over nothing: rk_4_main and the assembled RHS return without
+    writing, and adapt_dt_cfl returns its initial bound 1e14
+    (mod_time_ops.f90:27-29), the minimum over no elements."""
+    D = Dev()
+    tp = tab_ptrs(D)
+    sentinel = 7.25
+    zin = torch.full((64,), sentinel, dtype=torch.float64, device="cuda")
+    zout = torch.full((64,), sentinel, dtype=torch.float64, device="cuda")
+    zi = torch.ones(64, dtype=torch.int32, device="cuda")
+    i, o, c = zin.data_ptr(), zout.data_ptr(), zi.data_ptr()
+    _lib.call("sod_rk_4_main", 1, 0, 0, 0, 0, 0, 0, None, None, tp["dlx"], tp["xgp"], None, tp["inv"], tp["gI"],
+              tp["gJ"], tp["gK"], 1, c, tp["Ngp"], tp["dNgp"], i, tp["wgp"], i, i, i, 1e-3, i, i, 287.0, 1.4,
+              1004.5, 0.71, *[o] * 18, c, o, i, o)
+    _lib.call("sod_full_rhs", 0, 0, 0, c, c, *[i] * 9, 287.0, 1.4, 1004.5, 0.71, *[o] * 4)
+    dt = ctypes.c_double(0.0)
+    cd = ctypes.c_double(0.5)
+    _lib.call("sod_adapt_dt_cfl", 0, 0, c, i, i, i, 0.5, ctypes.addressof(dt), ctypes.addressof(cd), i, i, i)
+    torch.cuda.synchronize()
+    assert dt.value == 1e14
+    assert bool((zin == sentinel).all()) and bool((zout == sentinel).all())
