@@ -93,3 +93,22 @@ def test_stage_configuration_arguments_are_checked():
         _lib.set_stage_config(nw, atomic)
     for mode in (1, 2, 0):
         _lib.set_autotune(mode)
+
+
+def test_missing_library_fails_loudly():
+    """Without the HIP build every entry point raises: the package has no CPU
+    fallback.  Run in a child process so that this process keeps its loaded
+    library."""
+    import subprocess
+    import sys
+    code = ("import sod2d_refmap_amd._lib as L\n"
+            "L.LIB_PATH = '/nonexistent/libsod2d_hip.so'\n"
+            "L._lib = None\n"
+            "from sod2d_refmap_amd.solver import PeriodicSolver\n"
+            "try:\n"
+            "    L.call('sod_init', 0)\n"
+            "except RuntimeError as e:\n"
+            "    print('raised:', e)\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "raised:" in r.stdout and "is not built" in r.stdout
