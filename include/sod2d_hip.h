@@ -83,6 +83,21 @@ int sod_update_device(const void *host, size_t bytes);
 int sod_update_host(void *host, size_t bytes);
 void *sod_deviceptr(const void *host);
 
+/* No reference counterpart (the reference re-reads every mesh array on every
+ * call).  The drop-in entry points keep derived copies of the static mesh
+ * arrays they receive (connec, lpoin_w, He, gpvol, Ml in lane layout with
+ * an inverse connectivity, and a per-node minimum of helem), keyed on the
+ * arrays' device addresses.  A caller that rewrites a mesh array in place at
+ * the same address calls this before the next call: every derived copy is
+ * dropped and rebuilt on first use.  sod_update_device and sod_exit_data do
+ * it by themselves for the mesh arrays inside the block they touch.  Without
+ * it the library still compares a sampled fingerprint of the arrays (4096
+ * sampled elements and working nodes) and a hash of all of connec and helem
+ * (CFL step) on every call, which catches most but not every in-place
+ * change of He or gpvol.  Time loops (sod_time_loop_*) hold their own copy
+ * for their lifetime and are not affected. */
+int sod_invalidate_mesh_cache(void);
+
 /* Named kernel timing with HIP events on the library stream; the
  * counterpart of the reference's NVTX ranges (mod_nvtx.f90:50-101).
  * sod_profile_get returns the summed duration and count of the ranges
@@ -201,7 +216,14 @@ int sod_volAvg_EK(int nelem, int npoin, const int *connec, const double *gpvol,
  * Rmass/Rmom/Rener = R_conv + R_diff, Reta = generic_scalar_convec_ijk of
  * (u eta, eta, u).  No single reference routine computes this sum; it is the
  * unit the fused stage kernel evaluates, exposed for parity tests and for
- * the adjoint. */
+ * the adjoint.  The numerical parameters select the stage's branches as in
+ * rk_4_main: flag_total_enthalpy != 0 takes full_convec_ijk_H with E := the
+ * conserved E (time_integ.f90:361-362); with the Sutherland gate on
+ * (flag_real_diff == 1 and flag_diff_suth == 1, time_integ.f90:312-316)
+ * mu_factor is Sutherland's factor and the fluid viscosity is formed from the
+ * stage temperature, with the gate off the array passed as mu_factor is the
+ * fluid viscosity mu_fluid itself, as the stages then read the caller's
+ * mu_fluid unchanged. */
 int sod_full_rhs(int nelem, int npoin, int npoin_w, const int *connec, const int *lpoin_w,
                  const double *He, const double *gpvol, const double *Ml, const double *rho,
                  const double *q, const double *E, const double *mu_factor, const double *mu_e,
@@ -213,7 +235,10 @@ int sod_full_rhs(int nelem, int npoin, int npoin_w, const int *connec, const int
  * returns the vector-Jacobian products
  *   g_rho = lam . dR/drho, g_q(npoin,3) = lam . dR/dq, g_E = lam . dR/dE
  * at the nodes and g_mu_e(nelem,64) = lam . dR/dmu_e at the element nodes,
- * with mu_factor and mu_sgs held fixed.  Outputs are overwritten. */
+ * with mu_factor and mu_sgs held fixed (with the Sutherland gate off the
+ * mu_factor argument is mu_fluid, as for sod_full_rhs, and is held fixed).
+ * Outputs are overwritten.  flag_total_enthalpy != 0 is refused with
+ * SOD_ERR_UNSUPPORTED: the adjoint of full_convec_ijk_H is not implemented. */
 int sod_full_rhs_vjp(int nelem, int npoin, int npoin_w, const int *connec, const int *lpoin_w,
                      const double *He, const double *gpvol, const double *Ml, const double *rho,
                      const double *q, const double *E, const double *mu_factor, const double *mu_e,

@@ -33,6 +33,8 @@ class OrcParams(ctypes.Structure):
         ("Rgas_inf", ctypes.c_double),
         ("flag_normalise_entropy", ctypes.c_int),
         ("flag_total_enthalpy", ctypes.c_int),
+        ("flag_real_diff", ctypes.c_int),
+        ("flag_diff_suth", ctypes.c_int),
     ]
 
 

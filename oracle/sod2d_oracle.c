@@ -81,6 +81,8 @@ void orc_default_params(orc_params *p)
     p->Rgas_inf = 1004.0 * (1.4 - 1.0) / 1.4;
     p->flag_normalise_entropy = 1;
     p->flag_total_enthalpy = 0;
+    p->flag_real_diff = 1;   /* mod_numerical_params.f90:93 */
+    p->flag_diff_suth = 1;   /* :94 */
 }
 
 /* ------------------------------------------------------------------ A1 -- */
@@ -1121,8 +1123,10 @@ void orc_rk_4_main(int nelem, int npoin, int npoin_w, const int *connec, const i
         orc_generic_scalar_convec_ijk(nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                       gmshAtoI, gmshAtoJ, gmshAtoK, f_eta, aux_eta, aux_u, Reta);
         orc_lumped_solver_scal(npoin, npoin_w, lpoin_w, Ml, Reta);
-        /* time_integ.f90:311-316 */
-        orc_sutherland_viscosity(npoin, aux_Tem, mu_factor, mu_fluid);
+        /* time_integ.f90:311-316: with the gate off mu_fluid is the caller's,
+         * unchanged through the step */
+        if (prm->flag_real_diff == 1 && prm->flag_diff_suth == 1)
+            orc_sutherland_viscosity(npoin, aux_Tem, mu_factor, mu_fluid);
         /* time_integ.f90:321 */
         orc_full_diffusion_ijk(nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK, gmshAtoI,
                                gmshAtoJ, gmshAtoK, Cp, Prt, aux_rho, aux_rho, aux_u, aux_Tem,
@@ -1183,7 +1187,8 @@ void orc_rk_4_main(int nelem, int npoin, int npoin_w, const int *connec, const i
         Reta[n] = -Reta_sum[n];
     }
     /* time_integ.f90:482-493 */
-    orc_sutherland_viscosity(npoin, Tem, mu_factor, mu_fluid);
+    if (prm->flag_real_diff == 1 && prm->flag_diff_suth == 1)
+        orc_sutherland_viscosity(npoin, Tem, mu_factor, mu_fluid);
     orc_smart_visc_spectral(nelem, npoin, npoin_w, connec, lpoin_w, Reta, gpvol, wgp, gamma_gas,
                             rho, u, csound, Tem, eta, helem_l, helem, invAtoIJK, prm, mu_e,
                             mue_l);

@@ -44,6 +44,8 @@ typedef struct {
     double Rgas_inf;              /* nscbc_Rgas_inf, :45 (set by the case) */
     int flag_normalise_entropy;   /* :23 */
     int flag_total_enthalpy;      /* :20 */
+    int flag_real_diff;           /* :93 */
+    int flag_diff_suth;           /* :94 */
 } orc_params;
 
 void orc_default_params(orc_params *p);

@@ -47,6 +47,7 @@ _SIGS = {
     "sod_exit_data": [P, c_int],
     "sod_update_device": [P, c_size_t],
     "sod_update_host": [P, c_size_t],
+    "sod_invalidate_mesh_cache": [],
     "sod_profile_enable": [c_int],
     "sod_profile_reset": [],
     "sod_profile_get": [ctypes.c_char_p, ctypes.POINTER(c_double), ctypes.POINTER(c_int)],

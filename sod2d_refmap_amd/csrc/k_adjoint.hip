@@ -83,7 +83,9 @@ __global__ __launch_bounds__(64 * AWPB) __attribute__((amdgpu_waves_per_eu(2, 8)
     const double eta = (r * P.ig1) * lphi;
     const double kk = 0.5 * umag;
     const double sT = sqrt(T);
-    const double muf = mf * 0.000001458 * (T * sT) / (T + 110.4);
+    // Sutherland's law, or the caller's mu_fluid (a constant of the state)
+    // with the gate off
+    const double muf = P.suth ? mf * 0.000001458 * (T * sT) / (T + 110.4) : mf;
     double hm[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -426,7 +428,7 @@ __global__ __launch_bounds__(64 * AWPB) __attribute__((amdgpu_waves_per_eu(2, 8)
     {
         double LT = Lp_T, Lr = Lp_r, Lp = Lp_p, Lri = L_ri, Lk = Lp_k;
         const double den = T + 110.4;
-        LT += L_muf * mf * 0.000001458 * (1.5 * sT * den - T * sT) / (den * den);
+        if (P.suth) LT += L_muf * mf * 0.000001458 * (1.5 * sT * den - T * sT) / (den * den);
         Lr += Lp_eta * P.ig1 * (lphi - P.gamma);
         Lp += Lp_eta * r * P.ig1 / p;
         Lp += LT * ri * P.iRgas + Lp_h * P.gog1 * ri;
@@ -749,6 +751,9 @@ extern "C" int sod_full_rhs_vjp(int nelem, int npoin, int npoin_w, const int* co
     SOD_TRY(ensure_init());
     if (nelem < 0 || npoin < 0 || npoin_w < 0 || npoin_w > npoin)
         return fail(SOD_ERR_ARG, "full_rhs_vjp: inconsistent sizes");
+    if (rt().params.flag_total_enthalpy != 0)
+        return fail(SOD_ERR_UNSUPPORTED,
+                    "full_rhs_vjp: the adjoint of full_convec_ijk_H (flag_total_enthalpy) is not implemented");
     const int *dc, *dl;
     const double *dHe, *dgv, *dMl, *dr, *dq, *dE, *dmf, *dmue, *dsgs, *dlm, *dlq, *dle, *dlt;
     double *dgr, *dgq, *dgE, *dgmu;
@@ -785,7 +790,7 @@ extern "C" int sod_full_rhs_vjp(int nelem, int npoin, int npoin_w, const int* co
     SOD_HIP(hipMemcpyAsync(lam + 4 * np, dle, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
     SOD_HIP(hipMemcpyAsync(lam + 5 * np, dlt, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
     const sod_numerical_params& prm = rt().params;
-    const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
+    const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm);
     SOD_PROF("full_rhs_vjp");   // the element kernel, the mu_e transpose and the node sums
     if (nelem > 0) {
         if (lm.has_sgs)

@@ -47,7 +47,13 @@ constexpr int BLK = 256;
 // into six node arrays (stride rstride) with device-scope atomics, as the
 // reference scatters them (elem_convec.f90:442-455, elem_diffu.f90:183-193);
 // otherwise they are stored per element node into relem.
-template <bool HAS_SGS, int NW, bool ATOMIC>
+// TOTAL_H: the convective energy residual of full_convec_ijk_H
+// (elem_convec.f90:20-208, selected by flag_total_enthalpy,
+// time_integ.f90:361-362) with the total enthalpy per volume H = E + p
+// instead of the h + k split of full_convec_ijk.
+// mufac is mu_factor with the Sutherland gate on (P.suth) and the caller's
+// mu_fluid otherwise.
+template <bool HAS_SGS, int NW, bool ATOMIC, bool TOTAL_H>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_stage_rhs(
     int nelem, const int* __restrict__ conn, const double* __restrict__ He,
     const double* __restrict__ gpv, const double* __restrict__ mue, const double* __restrict__ musgs,
@@ -94,8 +100,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
     // logarithm and a power (same value to a few ulps; pow costs ~260 instructions)
     const double eta = (r * P.ig1) * (log(p) - P.gamma * log(r));
     const double kk = 0.5 * umag;
-    // Sutherland's law (mod_fluid_viscosity.f90:35-39), T^1.5 as T*sqrt(T)
-    const double muf = mf * 0.000001458 * (T * sqrt(T)) / (T + 110.4);
+    // Sutherland's law (mod_fluid_viscosity.f90:35-39), T^1.5 as T*sqrt(T),
+    // or the caller's mu_fluid with the gate off (time_integ.f90:312-316)
+    const double muf = P.suth ? mf * 0.000001458 * (T * sqrt(T)) / (T + 110.4) : mf;
     double hm[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -107,7 +114,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
 
     double(*S)[64] = sh[w];
     const int me = swz(i, j, k);
-    const double hk = h + kk;
+    // the convected energy variable: h + k (full_convec_ijk), or H = E + p
+    // (full_convec_ijk_H: fe = (E+p)u, gradE = grad(p+E))
+    const double hk = TOTAL_H ? en + p : h + kk;
     S[P_RHO][me] = r;
     S[P_U0][me] = u0;
     S[P_U1][me] = u1;
@@ -127,6 +136,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
     //   momentum u_a,b (t_b + rho DU_b + sA_b)
     //   entropy  eta_b (DU_b + DA_b)
     // (E := h in the energy flux, time_integ.f90:364; only h + k enters).
+    // With TOTAL_H the energy residual has the entropy's form in H:
+    //   energy   H_b (DU_b + DA_b)   [div((E+p)u) and u.grad(E+p)]
     double divQ = 0, accE = 0, accEta = 0, accQ0 = 0, accQ1 = 0, accQ2 = 0;
     double gP[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, gR[3] = {0, 0, 0};
     double gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};   // gU[a][b] = du_a/dx_b
@@ -162,7 +173,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
             accQ0 += u0b * Y;
             accQ1 += u1b * Y;
             accQ2 += u2b * Y;
-            accE += hkb * (Y + r * DA);
+            accE += TOTAL_H ? hkb * (DU + DA) : hkb * (Y + r * DA);
             accEta += etab * (DU + DA);
             divQ += t;
         }
@@ -184,7 +195,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
     const double Arho = u0 * gR[0] + u1 * gR[1] + u2 * gR[2];
     const double accQ[3] = {accQ0, accQ1, accQ2};
     double Rm = 0.5 * (divQ + r * divU) + 0.5 * Arho;
-    double Re = 0.25 * (accE + hk * (divQ + r * divU + Arho));
+    // full_convec_ijk_H: 1/2 (div(fe) + (E+p) div u + u.grad(E+p))
+    double Re = TOTAL_H ? 0.5 * (accE + hk * divU) : 0.25 * (accE + hk * (divQ + r * divU + Arho));
     double Rq[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -331,6 +343,10 @@ struct StepState {
     unsigned long long* dt_bits;
     double cfl_conv, cfl_diff;
     int has_diff;
+    // Sutherland gate (time_integ.f90:482-486): the final update writes
+    // mu_fluid from the new temperature; otherwise mu_fluid is the caller's
+    // and is only read
+    int suth;
 };
 
 constexpr int RED_BLOCKS = 1024;
@@ -420,8 +436,13 @@ __global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, co
             // entropy residual Reta = -Reta_sum (time_integ.f90:467-471),
             // kept in the (now free) first aux array for the entropy viscosity
             st.aux[n] = -Rs[R_ETA];
-            const double mu = sutherland(st.mu_factor[n], T);
-            st.mu_fluid[n] = mu;
+            double mu;
+            if (st.suth) {
+                mu = sutherland(st.mu_factor[n], T);
+                st.mu_fluid[n] = mu;
+            } else {
+                mu = st.mu_fluid[n];
+            }
             if (!BATCHED && st.hmin != nullptr) {
                 const double h = st.hmin[n];
                 if (h < HUGE_VAL)
@@ -678,9 +699,14 @@ __global__ __launch_bounds__(256) void k_smart_visc_lane(
 
 // Position-weighted fingerprint of sampled mesh data (connec, He, gpvol,
 // lpoin_w, Ml) so that a prepared mesh is rebuilt when a caller reuses the
-// same addresses for different contents.  Workgroup r (16 of them) samples
-// round r and leaves its 4 sums in out[4r .. 4r+3]; the host adds the rounds
-// in order (fingerprint()).
+// same addresses for different contents.  The 4096 samples k = t + 256 r
+// (workgroup r of 16) take 4096 evenly spaced elements, each at a node and
+// an He entry that vary with k, and 4096 evenly spaced lpoin_w entries.
+// Workgroup r leaves its 4 sums in out[4r .. 4r+3]; the host adds the
+// rounds in order (fingerprint()).  This is a safety net for meshes
+// rewritten in place, not a proof: callers that change mesh arrays at the
+// same addresses say so with sod_invalidate_mesh_cache (sod_update_device
+// and sod_exit_data of a registered mesh array do it for them).
 __global__ __launch_bounds__(256) void k_fingerprint(int nelem, int npoin_w, const int* __restrict__ connec,
                                                      const double* __restrict__ He,
                                                      const double* __restrict__ gpvol,
@@ -688,12 +714,14 @@ __global__ __launch_bounds__(256) void k_fingerprint(int nelem, int npoin_w, con
                                                      const double* __restrict__ Ml, double* __restrict__ out) {
     __shared__ double s[4][256];
     const int t = threadIdx.x, smp = blockIdx.x;
-    const int e = (int)(((long)nelem * smp + (t >> 4)) / 16) % nelem;
-    const int nd = t & 63;
-    const double wgt = 1.0 + 0.001 * (t + 256 * smp);
-    const int w = (int)(((long)npoin_w * (t + 256 * smp)) / 4096) % npoin_w;
+    const long k = t + 256L * smp;
+    const int e = (int)(((long)nelem * k) / 4096);
+    const int nd = (int)((k * 37 + (k >> 6)) & 63);
+    const int hi = (int)((k * 131 + (k >> 4)) % 576);
+    const double wgt = 1.0 + 0.001 * (double)k;
+    const int w = (int)(((long)npoin_w * k) / 4096);
     s[0][t] = wgt * connec[(size_t)e + (size_t)nelem * nd];
-    s[1][t] = wgt * He[576 * (size_t)e + (t % 576)];
+    s[1][t] = wgt * He[576 * (size_t)e + hi];
     s[2][t] = wgt * gpvol[64 * (size_t)e + nd];
     s[3][t] = wgt * lpoin_w[w] + wgt * Ml[lpoin_w[w] - 1];
     __syncthreads();
@@ -781,6 +809,30 @@ void fused_release() {
         sl.P.release();
         sl = PrepSlot();
     }
+}
+
+// Drops the drop-in entry points' prepared meshes built from an array that
+// lies in [lo, lo + bytes) (every one of them when lo is null), after the
+// work queued on the library stream.
+int fused_invalidate(const void* lo, size_t bytes) {
+    const char* a = static_cast<const char*>(lo);
+    bool synced = false;
+    for (auto& sl : g_slots) {
+        if (sl.used == 0) continue;
+        bool hit = lo == nullptr;
+        for (int t = 0; t < 5 && !hit; ++t) {
+            const char* k = static_cast<const char*>(sl.P.key[t]);
+            hit = k >= a && k < a + bytes;
+        }
+        if (!hit) continue;
+        if (!synced) {
+            SOD_HIP(hipStreamSynchronize(rt().stream));
+            synced = true;
+        }
+        sl.P.release();
+        sl = PrepSlot();
+    }
+    return SOD_OK;
 }
 
 // connectivity ids must lie in 1..npoin (the kernels gather and scatter
@@ -1031,31 +1083,41 @@ constexpr StageCfg kStageCfgs[] = {{1, false}, {2, false}, {4, false}, {4, true}
 constexpr int kNumStageCfgs = sizeof(kStageCfgs) / sizeof(kStageCfgs[0]);
 constexpr int kPinned = 16;   // autotune mode kPinned + c: configuration c pinned
 
-template <bool HS, int NW, bool AT>
+template <bool HS, int NW, bool AT, bool TH>
 static void launch_rhs_t(hipStream_t s, int nelem, const Prepared& P, const double* const in[6], long np,
                          const StageParams& sp) {
-    hipLaunchKernelGGL((k_stage_rhs<HS, NW, AT>), dim3(grid_for(nelem, NW)), dim3(64 * NW), 0, s, nelem, P.conn,
-                       P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2], in[3], in[4], in[5],
+    hipLaunchKernelGGL((k_stage_rhs<HS, NW, AT, TH>), dim3(grid_for(nelem, NW)), dim3(64 * NW), 0, s, nelem,
+                       P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2], in[3], in[4], in[5],
                        AT ? P.rn : P.relem, np, sp);
 }
 
-// stage kernel of configuration `cfg` on the state in[] = rho, q0, q1, q2, E, mu_factor
-template <bool HS>
+// stage kernel of configuration `cfg` on the state in[] = rho, q0, q1, q2, E
+// and the per-node viscosity input (mu_factor, or mu_fluid with the
+// Sutherland gate off)
+template <bool HS, bool TH>
 static void launch_rhs_hs(int cfg, hipStream_t s, int nelem, const Prepared& P, const double* const in[6], long np,
                           const StageParams& sp) {
     switch (cfg) {
-        case 1: launch_rhs_t<HS, 2, false>(s, nelem, P, in, np, sp); break;
-        case 2: launch_rhs_t<HS, 4, false>(s, nelem, P, in, np, sp); break;
-        case 3: launch_rhs_t<HS, 4, true>(s, nelem, P, in, np, sp); break;
-        default: launch_rhs_t<HS, 1, false>(s, nelem, P, in, np, sp); break;
+        case 1: launch_rhs_t<HS, 2, false, TH>(s, nelem, P, in, np, sp); break;
+        case 2: launch_rhs_t<HS, 4, false, TH>(s, nelem, P, in, np, sp); break;
+        case 3: launch_rhs_t<HS, 4, true, TH>(s, nelem, P, in, np, sp); break;
+        default: launch_rhs_t<HS, 1, false, TH>(s, nelem, P, in, np, sp); break;
     }
 }
-static void launch_stage_rhs(int cfg, bool has_sgs, hipStream_t s, int nelem, const Prepared& P,
+// total_h: full_convec_ijk_H in place of full_convec_ijk (flag_total_enthalpy)
+static void launch_stage_rhs(int cfg, bool has_sgs, bool total_h, hipStream_t s, int nelem, const Prepared& P,
                              const double* const in[6], long np, const StageParams& sp) {
-    if (has_sgs)
-        launch_rhs_hs<true>(cfg, s, nelem, P, in, np, sp);
-    else
-        launch_rhs_hs<false>(cfg, s, nelem, P, in, np, sp);
+    if (has_sgs) {
+        if (total_h)
+            launch_rhs_hs<true, true>(cfg, s, nelem, P, in, np, sp);
+        else
+            launch_rhs_hs<true, false>(cfg, s, nelem, P, in, np, sp);
+    } else {
+        if (total_h)
+            launch_rhs_hs<false, true>(cfg, s, nelem, P, in, np, sp);
+        else
+            launch_rhs_hs<false, false>(cfg, s, nelem, P, in, np, sp);
+    }
 }
 
 template <bool FINAL, bool BATCHED>
@@ -1117,6 +1179,7 @@ static StepState step_state(Prepared& P, const FusedArgs& a) {
     st.dt_bits = nullptr;
     st.cfl_conv = st.cfl_diff = 0.0;
     st.has_diff = 0;
+    st.suth = sutherland_gate(rt().params) ? 1 : 0;
     return st;
 }
 
@@ -1134,7 +1197,7 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
         st.cfl_diff = fold->cfl_diff;
         st.has_diff = fold->has_diff;
     }
-    const StageParams sp = make_stage_params(a.gamma_gas, a.Rgas, a.Cp, a.Prt, prm.c_rho, prm.c_ener);
+    const StageParams sp = make_stage_params(a.gamma_gas, a.Rgas, a.Cp, a.Prt, prm);
     const int ugrid = std::min(RED_BLOCKS, std::max(1, grid_for(npoin_w, BLK)));
     for (int stage = 0; stage < 4; ++stage) {
         const double *ir, *i0, *i1, *i2, *iE;
@@ -1145,15 +1208,15 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
         }
         if (nelem > 0) {
             SOD_PROF("stage_rhs");
-            const double* const in[6] = {ir, i0, i1, i2, iE, a.mu_factor};
-            launch_stage_rhs(P.cfg, has_sgs, s, nelem, P, in, (long)np, sp);
+            const double* const in[6] = {ir, i0, i1, i2, iE, sp.suth ? a.mu_factor : a.mu_fluid};
+            launch_stage_rhs(P.cfg, has_sgs, prm.flag_total_enthalpy != 0, s, nelem, P, in, (long)np, sp);
         }
         SOD_PROF("stage_update");
         const dim3 grid(stage == 3 ? ugrid : node_grid(npoin_w));
         launch_update(stage == 3, B != nullptr, kStageCfgs[P.cfg].atomic, grid, s, npoin, npoin_w, a.lpoin_w, stage,
                       d_dt, a.gamma_gas, a.Rgas, st, B ? B->node_inst : nullptr);
     }
-    if (P.nslave > 0)
+    if (P.nslave > 0 && st.suth)
         hipLaunchKernelGGL(k_slave_sutherland, dim3(grid_for(P.nslave, BLK)), dim3(BLK), 0, s, P.nslave,
                            P.slaves, a.Tem, a.mu_factor, a.mu_fluid);
     // entropy viscosity for the next step (time_integ.f90:492-493)
@@ -1220,8 +1283,8 @@ static int choose_stage_cfg(Prepared& P, const FusedArgs& a, bool has_sgs, const
     }
     const StepState st = step_state(P, a);
     const sod_numerical_params& prm = rt().params;
-    const StageParams sp = make_stage_params(a.gamma_gas, a.Rgas, a.Cp, a.Prt, prm.c_rho, prm.c_ener);
-    const double* const in[6] = {a.rho, a.q, a.q + np, a.q + 2 * np, a.E, a.mu_factor};
+    const StageParams sp = make_stage_params(a.gamma_gas, a.Rgas, a.Cp, a.Prt, prm);
+    const double* const in[6] = {a.rho, a.q, a.q + np, a.q + 2 * np, a.E, sp.suth ? a.mu_factor : a.mu_fluid};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     SOD_HIP(hipEventCreate(&e0));
     hipError_t err = hipEventCreate(&e1);
@@ -1232,7 +1295,8 @@ static int choose_stage_cfg(Prepared& P, const FusedArgs& a, bool has_sgs, const
         double tmin = HUGE_VAL;
         for (int rep = 0; rep < 4 && err == hipSuccess; ++rep) {
             err = hipEventRecord(e0, s);
-            if (a.nelem > 0) launch_stage_rhs(c, has_sgs, s, a.nelem, P, in, (long)np, sp);
+            if (a.nelem > 0)
+                launch_stage_rhs(c, has_sgs, prm.flag_total_enthalpy != 0, s, a.nelem, P, in, (long)np, sp);
             launch_update(false, B != nullptr, kStageCfgs[c].atomic, dim3(node_grid(a.npoin_w)), s, a.npoin,
                           a.npoin_w, a.lpoin_w,
                           0, d_dt, a.gamma_gas, a.Rgas, st, B ? B->node_inst : nullptr);
@@ -1313,10 +1377,10 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
     SOD_TRY(load_viscosities(Pm, nelem, mu_e, mu_sgs, &has_sgs));
     SOD_PROF("full_rhs");   // the element kernel and the node sums
     if (nelem > 0) {
-        const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm.c_rho, prm.c_ener);
+        const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm);
         // the default configuration (element-node stores, summed below)
         const double* const in[6] = {rho, q, q + np, q + 2 * np, E, mu_factor};
-        launch_stage_rhs(0, has_sgs != 0, s, nelem, Pm, in, (long)np, sp);
+        launch_stage_rhs(0, has_sgs != 0, prm.flag_total_enthalpy != 0, s, nelem, Pm, in, (long)np, sp);
     }
     if (npoin > 0)
         hipLaunchKernelGGL(k_assemble_nodes, dim3(std::max(1, grid_for(npoin, 256))), dim3(256), 0, s, npoin,

@@ -292,38 +292,51 @@ __global__ __launch_bounds__(256) void k_adapt_dt_nodes(int npoin, const double*
 }
 
 // Whether the cached h_min of the drop-in CFL entry point still belongs to
-// (connec, helem): a position-weighted fingerprint of 4096 sampled entries
-// of each is compared with the stored one; *stale = 1 when they differ (or
-// `force`), and the new fingerprint is stored.  One block of 256 threads.
-__global__ __launch_bounds__(256) void k_dt_cache_check(int nelem, const int* __restrict__ connec,
-                                                        const double* __restrict__ helem,
-                                                        double* __restrict__ fp, int* __restrict__ stale,
-                                                        int force) {
-    __shared__ double s_a[256], s_b[256];
-    const int t = threadIdx.x;
-    const long n = (long)nelem * 64;
-    double a = 0.0, b = 0.0;
-#pragma unroll   // the 16 rounds' loads issued together; the sums keep their order
-    for (int smp = 0; smp < 16; ++smp) {
-        const long k = t + 256L * smp;
-        const double w = 1.0 + 0.001 * (double)k;
-        a += w * (double)connec[(n * k) / 4096];
-        b += w * helem[((long)nelem * k) / 4096];
+// (connec, helem): every entry of both arrays enters an exact hash
+// (a 64-bit sum, mod 2^64, of a mixing function of each value's bits and
+// position, so that the order of the additions does not matter).
+// k_dt_hash adds the hash of its share into acc[0] (connec) and acc[1]
+// (helem); k_dt_cache_check compares them with the stored fingerprint, sets
+// *stale (or `force`), stores them and clears acc for the next call.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+    // splitmix64 finaliser (Steele, Lea and Flood, OOPSLA 2014)
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ULL;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebULL;
+    x ^= x >> 31;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_dt_hash(long n_conn, const int* __restrict__ connec, int nelem,
+                                                 const double* __restrict__ helem,
+                                                 unsigned long long* __restrict__ acc) {
+    unsigned long long a = 0, b = 0;
+    const long stride = (long)gridDim.x * 256;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n_conn; t += stride)
+        a += mix64((unsigned long long)(unsigned)connec[t] ^ mix64((unsigned long long)t));
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < nelem; t += stride)
+        b += mix64((unsigned long long)__double_as_longlong(helem[t]) ^ mix64((unsigned long long)t + 0x9e37ULL));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
     }
-    s_a[t] = a;
-    s_b[t] = b;
-    __syncthreads();
-    if (t == 0) {
-        double sa = 0.0, sb = 0.0;
-        for (int q = 0; q < 256; ++q) {
-            sa += s_a[q];
-            sb += s_b[q];
-        }
-        const bool same = !force && fp[0] == sa && fp[1] == sb;
-        *stale = same ? 0 : 1;
-        fp[0] = sa;
-        fp[1] = sb;
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(acc, a);
+        atomicAdd(acc + 1, b);
     }
+}
+
+__global__ void k_dt_cache_check(unsigned long long* __restrict__ acc, unsigned long long* __restrict__ fp,
+                                 int* __restrict__ stale, int force) {
+    if (threadIdx.x != 0) return;
+    const bool same = !force && fp[0] == acc[0] && fp[1] == acc[1];
+    *stale = same ? 0 : 1;
+    fp[0] = acc[0];
+    fp[1] = acc[1];
+    acc[0] = 0;
+    acc[1] = 0;
 }
 
 // flag = 1 if any of the n values is non-zero
@@ -697,7 +710,7 @@ struct DtNodeCache {
     const double* helem = nullptr;
     int nelem = -1, npoin = -1;
     double* hmin = nullptr;
-    double* fp = nullptr;   // stored fingerprint (2)
+    unsigned long long* fp = nullptr;   // stored hashes (2), then the running sums (2)
     int* flags = nullptr;   // [0] cache stale, [1] mu_sgs holds a non-zero value
     void release() {
         (void)hipFree(hmin);
@@ -710,6 +723,21 @@ struct DtNodeCache {
 static DtNodeCache g_dtc;
 
 void dt_cache_release() { g_dtc.release(); }
+
+// Drops the cached h_min when connec or helem lies in [lo, lo + bytes)
+// (always when lo is null), after the work queued on the library stream.
+int dt_cache_invalidate(const void* lo, size_t bytes) {
+    DtNodeCache& C = g_dtc;
+    if (C.hmin == nullptr) return SOD_OK;
+    const char* a = static_cast<const char*>(lo);
+    const char* c = reinterpret_cast<const char*>(C.connec);
+    const char* h = reinterpret_cast<const char*>(C.helem);
+    if (lo == nullptr || (c >= a && c < a + bytes) || (h >= a && h < a + bytes)) {
+        SOD_HIP(hipStreamSynchronize(rt().stream));
+        C.release();
+    }
+    return SOD_OK;
+}
 
 // adapt_dt_cfl through the node form when mu_sgs is absent or zero (decided
 // on the device), through the element form otherwise; result bits in
@@ -724,7 +752,8 @@ int launch_adapt_dt_cached(int nelem, int npoin, const int* connec, const double
         SOD_HIP(hipStreamSynchronize(s));
         C.release();
         SOD_HIP(hipMalloc(&C.hmin, sizeof(double) * (size_t)std::max(1, npoin)));
-        SOD_HIP(hipMalloc(&C.fp, sizeof(double) * 2));
+        SOD_HIP(hipMalloc(&C.fp, sizeof(unsigned long long) * 4));
+        SOD_HIP(hipMemsetAsync(C.fp, 0, sizeof(unsigned long long) * 4, s));
         SOD_HIP(hipMalloc(&C.flags, sizeof(int) * 2));
         C.connec = connec;
         C.helem = helem;
@@ -736,7 +765,9 @@ int launch_adapt_dt_cached(int nelem, int npoin, const int* connec, const double
     auto* hbits = reinterpret_cast<unsigned long long*>(C.hmin);
     hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, s, d_dt_bits, 100000000000000.0);
     if (nelem <= 0 || npoin <= 0) return SOD_OK;
-    hipLaunchKernelGGL(k_dt_cache_check, dim3(1), dim3(256), 0, s, nelem, connec, helem, C.fp, C.flags, force);
+    hipLaunchKernelGGL(k_dt_hash, dim3(std::min(1024, grid_for((long)nelem * 64, 256))), dim3(256), 0, s,
+                       (long)nelem * 64, connec, nelem, helem, C.fp + 2);
+    hipLaunchKernelGGL(k_dt_cache_check, dim3(1), dim3(64), 0, s, C.fp + 2, C.fp, C.flags, force);
     hipLaunchKernelGGL(k_fill_u64, dim3(std::min(1024, grid_for(npoin, 256))), dim3(256), 0, s, (long)npoin, hbits,
                        kInfBits, C.flags);
     hipLaunchKernelGGL(k_hmin_build, dim3(std::min(2048, grid_for((long)nelem * 64, 256))), dim3(256), 0, s, nelem,

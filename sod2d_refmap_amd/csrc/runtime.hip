@@ -16,6 +16,8 @@ namespace sod {
 int upload_tables();   // tables.hip
 void fused_release();     // k_fused.hip
 void dt_cache_release();  // k_misc.hip
+int fused_invalidate(const void* lo, size_t bytes);      // k_fused.hip
+int dt_cache_invalidate(const void* lo, size_t bytes);   // k_misc.hip
 
 namespace {
 thread_local char g_err[2048] = "";
@@ -277,6 +279,10 @@ int sod_exit_data(const void* host, int copyout) {
                           hipMemcpyDeviceToHost));
     }
     SOD_HIP(hipStreamSynchronize(rt().stream));
+    // mesh data cached from this block must not outlive it (its device
+    // address may be handed out again)
+    SOD_TRY(fused_invalidate(it->second.dev, it->second.bytes));
+    SOD_TRY(dt_cache_invalidate(it->second.dev, it->second.bytes));
     SOD_HIP(hipFree(it->second.dev));
     g_present.erase(it);
     return SOD_OK;
@@ -299,6 +305,16 @@ int sod_update_device(const void* host, size_t bytes) {
     SOD_TRY(find_range(host, bytes, &d));
     SOD_HIP(hipStreamSynchronize(rt().stream));
     SOD_HIP(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice));
+    // new contents of a mesh array: its prepared copies are rebuilt
+    SOD_TRY(fused_invalidate(d, bytes));
+    SOD_TRY(dt_cache_invalidate(d, bytes));
+    return SOD_OK;
+}
+
+int sod_invalidate_mesh_cache(void) {
+    SOD_TRY(ensure_init());
+    SOD_TRY(fused_invalidate(nullptr, 0));
+    SOD_TRY(dt_cache_invalidate(nullptr, 0));
     return SOD_OK;
 }
 

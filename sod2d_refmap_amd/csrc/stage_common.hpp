@@ -65,13 +65,25 @@ __device__ __forceinline__ void wave_sync() {
 
 // Per-stage constants; ratios of the gas constants are formed once on the
 // host so that the element kernels divide only by node values.
+// suth: Sutherland's law is evaluated at each stage from the stage
+// temperature (flag_real_diff == 1 and flag_diff_suth == 1,
+// time_integ.f90:312-316); otherwise the stages read the caller's mu_fluid
+// unchanged, and the per-node viscosity input of the element kernels is
+// mu_fluid itself instead of mu_factor.
 struct StageParams {
     double gamma, gm1, ig1, gog1, iRgas, CpPr, Cp09, c_rho, c_ener;
+    int suth;
 };
 
-inline StageParams make_stage_params(double gamma, double Rgas, double Cp, double Pr, double c_rho,
-                                     double c_ener) {
+inline bool sutherland_gate(const sod_numerical_params& prm) {
+    return prm.flag_real_diff == 1 && prm.flag_diff_suth == 1;
+}
+
+inline StageParams make_stage_params(double gamma, double Rgas, double Cp, double Pr,
+                                     const sod_numerical_params& prm) {
+    const double c_rho = prm.c_rho, c_ener = prm.c_ener;
     StageParams P;
+    P.suth = sutherland_gate(prm) ? 1 : 0;
     P.gamma = gamma;
     P.gm1 = gamma - 1.0;
     P.ig1 = 1.0 / (gamma - 1.0);

@@ -24,3 +24,21 @@ def gpu():
     from sod2d_refmap_amd.device import init
     init(0)
     return torch.device("cuda:0")
+
+
+@pytest.fixture(autouse=True)
+def _library_defaults_after_gpu_test(request):
+    """Process-global library settings (numerical parameters, stage
+    configuration mode, rk_4_main implementation) go back to their defaults
+    after every GPU test, also when it fails, so that no test leaks its
+    settings into the next."""
+    yield
+    if request.node.get_closest_marker("gpu") is None:
+        return
+    from sod2d_refmap_amd import device
+    if not device._INITIALISED:
+        return
+    from sod2d_refmap_amd import _lib, cases
+    device.set_numerical_params(device.numerical_params(Rgas_inf=cases.GasParams().Rgas))
+    _lib.call("sod_set_autotune", 0)
+    _lib.call("sod_set_rk4_path", 0)

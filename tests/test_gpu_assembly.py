@@ -85,3 +85,55 @@ def test_prepared_mesh_cache_eviction(gpu):
             sols[i] = (m, sol, nxt)
     for _, sol, _ in sols:
         sol.close()
+
+
+def test_mesh_rewritten_in_place(gpu):
+    """A mesh array rewritten at the same device address.  The He and gpvol
+    of an element the prepared-mesh fingerprint does not sample (element 2 of
+    17^3: the 4096 samples take elements floor(4913 k / 4096)) are changed
+    and sod_invalidate_mesh_cache announces it: the next step uses the new
+    geometry.  helem of one element is changed without any call: the CFL
+    step's cache hashes all of connec and helem and is rebuilt, so dt equals
+    the oracle's bit for bit."""
+    import torch
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    m = pmesh.periodic_box(17)
+    gas = cases.GasParams()
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    s = cases.tgv_initial_state(m, gas, perturb_seed=4, perturb_amp=2e-2)
+    g = mesh_geometry(m)
+    assert 2 not in {(m.nelem * k) // 4096 for k in range(4096)}
+    sol = PeriodicSolver(m, gas, s)
+    dt = sol.adapt_dt()
+    sol.rk_4_main(dt)   # builds the prepared mesh
+    sol.load_state(s)
+    e = 2
+    g2 = {k: v.copy(order="F") for k, v in g.items()}
+    g2["He"][:, :, :, e] *= 1.01
+    g2["gpvol"][:, :, e] *= 0.99
+    sol.He[576 * e: 576 * (e + 1)].mul_(1.01)
+    sol.gpvol[64 * e: 64 * (e + 1)].mul_(0.99)
+    torch.cuda.synchronize()
+    _lib.call("sod_invalidate_mesh_cache")
+    print("[assembly] in-place He/gpvol rewrite of element 2, 17^3", flush=True)
+    ref = _oracle_step(m, g2, gas, s, dt)
+    sol.rk_4_main(dt)
+    out = sol.state()
+    for k in ("rho", "q", "E", "mu_e"):
+        err = rel_l2(out[k], ref[k])
+        assert err < 1e-12, (k, err)
+    # CFL step: helem of one element shrinks in place, no call
+    sol.load_state(s)
+    e = 4321
+    h2 = g["helem"].copy()
+    h2[e] *= 0.25
+    sol.helem.copy_(torch.from_numpy(np.ascontiguousarray(g["helem"])).to("cuda"))
+    sol.adapt_dt()   # cached h_min of the original helem
+    sol.helem[e:e + 1].mul_(0.25)
+    torch.cuda.synchronize()
+    zero = np.zeros((m.nelem, 64), order="F")
+    want = oracle.adapt_dt_cfl(m.connec, h2, s["u"], s["csound"], 0.5, 0.5, s["mu_fluid"], zero, s["rho"])
+    old = oracle.adapt_dt_cfl(m.connec, g["helem"], s["u"], s["csound"], 0.5, 0.5, s["mu_fluid"], zero, s["rho"])
+    assert want < old
+    assert sol.adapt_dt() == want
+    sol.close()

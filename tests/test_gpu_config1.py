@@ -1,0 +1,96 @@
+"""The reference's CPU case run on the GPU and checked against the oracle:
+the compressible Taylor-Green vortex in the periodic cube [0, 2pi]^3 on 8^3
+straight HEX64 elements (24^3 working nodes), FP64, gamma 1.4, Pr 0.71,
+Re 1600, Mach 0.1, rho0 1, the reference's initial condition with its real(4)
+coordinates (TGVSolver.f90:87-185), entropy viscosity on, Sutherland
+viscosity on, dt from adapt_dt_cfl at CFL 0.5 for both limits, 10 RK4 steps
+(the time loop of CFDSolverBase.f90:1501-1694: rk_4_main, time += dt,
+adapt_dt_cfl).
+
+After every step the level-2 state, the entropy viscosity, dt and the
+kinetic energy volAvg_EK are compared with the oracle stepping the same case:
+relative L2 <= 1e-10 per field over the 10 steps (<= 1e-12 after the first),
+dt and EK to 1e-12 relative.  Both the eager entry points (sod_rk_4_main +
+sod_adapt_dt_cfl + sod_volAvg_EK) and the graph-captured device loop
+(sod_time_loop_run, one replay per step) are checked.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from _util import mesh_geometry, rel_l2
+from sod2d_refmap_amd import cases, mesh as pmesh
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("rho", "q", "E", "u", "pr", "Tem", "eta", "csound", "machno", "mu_fluid", "mu_e", "mue_l")
+T = oracle.hex64_tables()
+NSTEPS = 10
+
+
+@pytest.fixture(scope="module")
+def tgv8():
+    m = pmesh.periodic_box(8)
+    g = mesh_geometry(m)
+    gas = cases.GasParams(Re=1600.0, Mach=0.1)
+    s = cases.tgv_initial_state(m, gas)
+    # the oracle's trajectory: state, dt and EK after every step
+    p = oracle.default_params(Rgas_inf=gas.Rgas)
+    zero = np.zeros((m.nelem, 64), order="F")
+    ref = {k: v.copy(order="F") for k, v in s.items()}
+
+    def dt_of(st):
+        return oracle.adapt_dt_cfl(m.connec, g["helem"], st["u"], st["csound"], 0.5, 0.5, st["mu_fluid"], zero,
+                                   st["rho"])
+
+    dt = dt_of(ref)
+    traj = []
+    for _ in range(NSTEPS):
+        used = dt
+        oracle.rk_4_main(m.connec, m.lpoin_w, g["He"], g["gpvol"], T, g["Ml"], g["helem"], g["helem_l"], used,
+                         gas.Rgas, gas.gamma_gas, gas.Cp, gas.Prt, ref["mu_factor"], ref, p)
+        ek = oracle.volAvg_EK(m.connec, g["gpvol"], T.Ngp, gas.rho0, ref["rho"], ref["u"])
+        dt = dt_of(ref)
+        traj.append(({k: v.copy() for k, v in ref.items()}, used, dt, ek))
+    print(f"[config 1] TGV 8^3: nelem={m.nelem} npoin_w={m.npoin_w} dt0={traj[0][1]:.6e} "
+          f"EK10={traj[-1][3]:.12f}", flush=True)
+    return m, g, gas, s, dt_of(s), traj
+
+
+def _check(step, out, ref, tol):
+    for k in KEYS:
+        err = rel_l2(out[k], ref[k])
+        assert err < tol, (step, k, err)
+
+
+def test_config1_eager_vs_oracle(gpu, tgv8):
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    m, g, gas, s, dt0, traj = tgv8
+    sol = PeriodicSolver(m, gas, s, cfl_conv=0.5, cfl_diff=0.5)
+    assert sol.adapt_dt() == dt0
+    for i, (ref, used, dt_next, ek) in enumerate(traj):
+        # the same dt sequence: bit-exact dt on the first step, then within roundoff
+        assert abs(sol.dt / used - 1) < 1e-12, (i, sol.dt, used)
+        sol.rk_4_main(sol.dt)
+        sol.time += sol.dt
+        _check(i, sol.state(), ref, 1e-12 if i == 0 else 1e-10)
+        assert abs(sol.volAvg_EK() / ek - 1) < 1e-12, i
+        sol.adapt_dt()
+        assert abs(sol.dt / dt_next - 1) < 1e-12, (i, sol.dt, dt_next)
+    print(f"  eager: time={sol.time:.9e} EK={traj[-1][3]:.12f}", flush=True)
+
+
+def test_config1_time_loop_vs_oracle(gpu, tgv8):
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    m, g, gas, s, dt0, traj = tgv8
+    sol = PeriodicSolver(m, gas, s, cfl_conv=0.5, cfl_diff=0.5)
+    sol.adapt_dt()
+    time = 0.0
+    for i, (ref, used, dt_next, ek) in enumerate(traj):
+        eks = sol.run_device(1, istep0=i + 1, log_first=1, log_every=1, keep_levels=True)
+        time += used
+        _check(i, sol.state(), ref, 1e-12 if i == 0 else 1e-10)
+        assert abs(eks[0] / ek - 1) < 1e-12, (i, eks[0], ek)
+        assert abs(sol.dt / dt_next - 1) < 1e-12, (i, sol.dt, dt_next)
+        assert abs(sol.time / time - 1) < 1e-13
+    sol.close()

@@ -1,17 +1,19 @@
 """The device time loop (sod_time_loop_*): a graph-captured step replayed
-with dt, time, step counter and EK history on the device, against the same
-steps taken through the eager entry points (sod_rk_4_main, sod_adapt_dt_cfl,
-sod_volAvg_EK), whose parity with the oracle is tested elsewhere.  The graph
-is replayed again after eager work on the same arrays, and every replay is
-checked."""
+with dt, time, step counter and EK history on the device, step by step
+against the oracle (rk_4_main, adapt_dt_cfl, volAvg_EK), and against the
+same steps taken through the eager entry points (sod_rk_4_main,
+sod_adapt_dt_cfl, sod_volAvg_EK).  The graph is replayed again after eager
+work on the same arrays, and every replay is checked."""
 import numpy as np
 import pytest
 
-from _util import rel_l2
+import oracle
+from _util import mesh_geometry, rel_l2
 from sod2d_refmap_amd import cases, mesh as pmesh
 from sod2d_refmap_amd.device import to_host
 
 pytestmark = pytest.mark.gpu
+T = oracle.hex64_tables()
 
 KEYS = ("rho", "q", "E", "u", "pr", "Tem", "eta", "csound", "machno", "mu_fluid", "mu_e", "mue_l")
 
@@ -81,3 +83,40 @@ def test_time_loop_fixed_dt_without_levels(gpu):
     for k in KEYS:
         assert rel_l2(sb[k], sa[k]) < 1e-12, k
     B.close()
+
+
+def test_time_loop_vs_oracle(gpu):
+    """Each replay of the captured step against the oracle's step of the
+    reference's loop (CFDSolverBase.f90:1501-1694): state, dt, time and EK."""
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    m = pmesh.periodic_box(3, curved_amp=0.05, perm_seed=13)
+    g = mesh_geometry(m)
+    gas = cases.GasParams()
+    s = cases.tgv_initial_state(m, gas, perturb_seed=5, perturb_amp=5e-2)
+    p = oracle.default_params(Rgas_inf=gas.Rgas)
+    zero = np.zeros((m.nelem, 64), order="F")
+    ref = {k: v.copy(order="F") for k, v in s.items()}
+
+    def dt_of(st):
+        return oracle.adapt_dt_cfl(m.connec, g["helem"], st["u"], st["csound"], 0.5, 0.5, st["mu_fluid"], zero,
+                                   st["rho"])
+
+    sol = PeriodicSolver(m, gas, s)
+    dt = dt_of(ref)
+    assert sol.adapt_dt() == dt
+    time = 0.0
+    for istep in range(1, 7):
+        eks = sol.run_device(1, istep0=istep, log_first=1, log_every=1, keep_levels=True)
+        oracle.rk_4_main(m.connec, m.lpoin_w, g["He"], g["gpvol"], T, g["Ml"], g["helem"], g["helem_l"], dt,
+                         gas.Rgas, gas.gamma_gas, gas.Cp, gas.Prt, ref["mu_factor"], ref, p)
+        time += dt
+        ek = oracle.volAvg_EK(m.connec, g["gpvol"], T.Ngp, gas.rho0, ref["rho"], ref["u"])
+        dt = dt_of(ref)
+        out = sol.state()
+        for k in KEYS:
+            err = rel_l2(out[k], ref[k])
+            assert err < (1e-12 if istep == 1 else 1e-10), (istep, k, err)
+        assert abs(eks[0] / ek - 1) < 1e-12, (istep, eks[0], ek)
+        assert abs(sol.dt / dt - 1) < 1e-12, (istep, sol.dt, dt)
+        assert abs(sol.time / time - 1) < 1e-13, istep
+    sol.close()
