@@ -31,8 +31,15 @@
  *                             sum_A R_A = 0 on periodic meshes, linear rho with
  *                             constant u -> R_mass = Ml u.grad(rho), an
  *                             independent dense-operator (Kronecker) evaluation
- *                             of the split form, and convergence of
- *                             R/Ml to the analytic flux divergence
+ *                             of the split form, convergence of
+ *                             R/Ml to the analytic flux divergence (for _H:
+ *                             to div((E+p)u)), and, with FP64 GLL tables on
+ *                             affine periodic meshes, the quadratic identities
+ *                             of the Kennedy-Gruber form (discrete kinetic
+ *                             energy and sum phi R_ener - phi^2/2 R_mass = 0
+ *                             with phi = h + k) and of the _H energy split,
+ *                             which the plain divergence form and transposed
+ *                             or regrouped terms fail (test_oracle_pins.py)
  *   orc_generic_scalar_convec_ijk  same family of checks
  *   orc_full_diffusion_ijk    constant/linear-u states -> 0, sum_A R_A = 0,
  *                             dense-operator weak form, convergence of -R/Ml
@@ -42,17 +49,22 @@
  *   orc_smart_visc_spectral   Reta = 0 -> mu_e = 0; saturated limiter -> the
  *                             filter reproduces cglob*beta_e exactly; a
  *                             brute-force 3x3x3 filter on a lattice field;
- *                             0 <= mu_e <= cglob*beta_e
+ *                             0 <= mu_e <= cglob*beta_e; closed forms of each
+ *                             limiter branch on uniform states (ce, the Mach
+ *                             limiter tanh(M^15 pi), the Jacobian limiter
+ *                             1 - (Jmin/Jmax)^2, saturation at beta_e) and of
+ *                             the normalisation over working nodes only
  *   orc_adapt_dt_cfl          closed form on a uniform state
  *   orc_volAvg_EK             closed form for TGV on the exact quadrature
  *   orc_rk_4_main             RK4 amplification polynomial on a linear
  *                             advection mode, conservation of mass/momentum/
  *                             energy, uniform state stays uniform
- * Parity unpinned (no reference output exists in this image): the exact
- * grouping of the split-form terms with h and k, the test-node evaluation of
- * kappa and nu_e, the mirror table of the entropy filter and the limiter
- * constants follow the reference text line by line and are checked only by
- * the invariants above.
+ * No reference output exists in this image; what the checks above leave
+ * open is the order of floating-point additions inside each routine (it
+ * follows the reference's loop nests line by line) and the mirror table's end
+ * entries of the entropy filter, which the brute-force filter test restates
+ * from the table mod_filters.f90:42-51 rather than from a mathematical
+ * property.
  */
 #include "sod2d_oracle.h"
 
