@@ -17,7 +17,20 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_OMP_LIB_PATH = os.path.join(_HERE, "liboracle_omp.so")
 _lib = None
+_use_omp = False
+
+
+def use_openmp(on: bool) -> None:
+    """Route every call of this module to liboracle_omp.so (the same source
+    built with OpenMP; its scatters are atomic, so its sums are reordered).
+    Only bench.py's multi-core CPU baseline does this; the checker is the
+    serial liboracle.so."""
+    global _lib, _use_omp
+    if on != _use_omp:
+        _use_omp = on
+        _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -41,10 +54,11 @@ class OrcParams(ctypes.Structure):
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+        path = _OMP_LIB_PATH if _use_omp else _LIB_PATH
+        if not os.path.exists(path):
             raise RuntimeError(
-                f"oracle library missing: {_LIB_PATH} (run `make -C oracle` or __graft_entry__.build())")
-        _lib = ctypes.CDLL(_LIB_PATH)
+                f"oracle library missing: {path} (run `make -C oracle` or __graft_entry__.build())")
+        _lib = ctypes.CDLL(path)
         _lib.orc_adapt_dt_cfl.restype = ctypes.c_double
         _lib.orc_volAvg_EK.restype = ctypes.c_double
         _lib.orc_invert_jacobian.restype = ctypes.c_double

@@ -75,6 +75,17 @@
 #define NN 64
 #define P1 4
 
+/* Built without OpenMP (liboracle.so, the checker) these are empty and every
+ * loop runs in the reference's order.  liboracle_omp.so, built with
+ * -fopenmp for the multi-core CPU baseline only, runs the element and node
+ * loops on all threads with atomic scatters (the reference's own assembly
+ * under OpenACC, elem_convec.f90:442-455), so its sums are reordered. */
+#ifdef _OPENMP
+#define ORC_PRAGMA(x) _Pragma(#x)
+#else
+#define ORC_PRAGMA(x)
+#endif
+
 /* Fortran-style accessors for the reference layouts (all 0-based here). */
 #define HE(i, j, g, e) He[(i) + 3 * (j) + 9 * (g) + 576 * (size_t)(e)]
 #define GPVOL(g, e) gpvol[(g) + 64 * (size_t)(e)]
@@ -470,6 +481,8 @@ void orc_full_convec_ijk(int nelem, int npoin, const int *connec, const double *
     double Re_mom[NN][3], Re_mass[NN], Re_ener[NN];
     double ul[NN][3], ql[NN][3], rhol[NN], prl[NN], El[NN], fel[NN][3], fl[NN][3][3],
         fuul[NN][3][3], REl[NN], fuel[NN][3], kl[NN], fkl[NN][3], fukl[NN][3], Rkl[NN];
+    ORC_PRAGMA(omp parallel for schedule(static) private(ip, Re_mom, Re_mass, Re_ener, ul, ql, rhol, prl, El,
+                                                         fel, fl, fuul, REl, fuel, kl, fkl, fukl, Rkl))
     for (int e = 0; e < nelem; ++e) {
         for (int a = 0; a < NN; ++a) ip[a] = CONNEC(e, a);
         for (int a = 0; a < NN; ++a) {
@@ -611,9 +624,14 @@ void orc_full_convec_ijk(int nelem, int npoin, const int *connec, const double *
             Re_ener[g] = GPVOL(g, e) * Re_ener[g];
         }
         for (int d = 0; d < 3; ++d)
-            for (int a = 0; a < NN; ++a) Rmom[ip[a] + np * d] += Re_mom[a][d];
+            for (int a = 0; a < NN; ++a) {
+                ORC_PRAGMA(omp atomic)
+                Rmom[ip[a] + np * d] += Re_mom[a][d];
+            }
         for (int a = 0; a < NN; ++a) {
+            ORC_PRAGMA(omp atomic)
             Rmass[ip[a]] += Re_mass[a];
+            ORC_PRAGMA(omp atomic)
             Rener[ip[a]] += Re_ener[a];
         }
     }
@@ -635,6 +653,8 @@ void orc_full_convec_ijk_H(int nelem, int npoin, const int *connec, const double
     double Re_mom[NN][3], Re_mass[NN], Re_ener[NN];
     double ul[NN][3], ql[NN][3], rhol[NN], prl[NN], El[NN], fel[NN][3], fl[NN][3][3],
         fuul[NN][3][3];
+    ORC_PRAGMA(omp parallel for schedule(static) private(ip, Re_mom, Re_mass, Re_ener, ul, ql, rhol, prl, El,
+                                                         fel, fl, fuul))
     for (int e = 0; e < nelem; ++e) {
         for (int a = 0; a < NN; ++a) ip[a] = CONNEC(e, a);
         for (int d = 0; d < 3; ++d)
@@ -734,9 +754,14 @@ void orc_full_convec_ijk_H(int nelem, int npoin, const int *connec, const double
             Re_ener[g] = GPVOL(g, e) * Re_ener[g];
         }
         for (int d = 0; d < 3; ++d)
-            for (int a = 0; a < NN; ++a) Rmom[ip[a] + np * d] += Re_mom[a][d];
+            for (int a = 0; a < NN; ++a) {
+                ORC_PRAGMA(omp atomic)
+                Rmom[ip[a] + np * d] += Re_mom[a][d];
+            }
         for (int a = 0; a < NN; ++a) {
+            ORC_PRAGMA(omp atomic)
             Rmass[ip[a]] += Re_mass[a];
+            ORC_PRAGMA(omp atomic)
             Rener[ip[a]] += Re_ener[a];
         }
     }
@@ -755,6 +780,7 @@ void orc_generic_scalar_convec_ijk(int nelem, int npoin, const int *connec, cons
     memset(Rconvec, 0, sizeof(double) * np);
     int ip[NN];
     double ul[NN][3], fel[NN][3], etal[NN], Re[NN];
+    ORC_PRAGMA(omp parallel for schedule(static) private(ip, ul, fel, etal, Re))
     for (int e = 0; e < nelem; ++e) {
         for (int a = 0; a < NN; ++a) ip[a] = CONNEC(e, a);
         for (int d = 0; d < 3; ++d)
@@ -793,7 +819,10 @@ void orc_generic_scalar_convec_ijk(int nelem, int npoin, const int *connec, cons
             for (int id = 0; id < 3; ++id) Re[g] = Re[g] + 0.5 * (ul[g][id] * gradE[id]);
             Re[g] = GPVOL(g, e) * Re[g];
         }
-        for (int a = 0; a < NN; ++a) Rconvec[ip[a]] += Re[a];
+        for (int a = 0; a < NN; ++a) {
+            ORC_PRAGMA(omp atomic)
+            Rconvec[ip[a]] += Re[a];
+        }
     }
 }
 
@@ -816,6 +845,8 @@ void orc_full_diffusion_ijk(int nelem, int npoin, const int *connec, const doubl
     int ip[NN];
     double ul[NN][3], rhol[NN], Teml[NN], mufluidl[NN], rhonl[NN];
     double tauXl[NN][3], tauYl[NN][3], tauZl[NN][3], gradTl[NN][3], gradRhol[NN][3], tauUl[NN][3];
+    ORC_PRAGMA(omp parallel for schedule(static) private(ip, ul, rhol, Teml, mufluidl, rhonl, tauXl, tauYl, tauZl,
+                                                         gradTl, gradRhol, tauUl))
     for (int e = 0; e < nelem; ++e) {
         for (int a = 0; a < NN; ++a) ip[a] = CONNEC(e, a);
         for (int a = 0; a < NN; ++a) {
@@ -901,9 +932,14 @@ void orc_full_diffusion_ijk(int nelem, int npoin, const int *connec, const doubl
                     divDm[2] = divDm[2] + wy * tauZl[qy][id];
                     divDm[2] = divDm[2] + wz * tauZl[qz][id];
                 }
+            ORC_PRAGMA(omp atomic)
             Rmass[ip[g]] += nu_e * divDr;
+            ORC_PRAGMA(omp atomic)
             Rener[ip[g]] += divDe;
-            for (int d = 0; d < 3; ++d) Rmom[ip[g] + np * d] += divDm[d];
+            for (int d = 0; d < 3; ++d) {
+                ORC_PRAGMA(omp atomic)
+                Rmom[ip[g] + np * d] += divDm[d];
+            }
         }
     }
 }
@@ -915,6 +951,7 @@ void orc_sutherland_viscosity(int npoin, const double *Tem, const double *mu_fac
                               double *mu_fluid)
 {
     const double C1 = 0.000001458, S = 110.4;
+    ORC_PRAGMA(omp parallel for schedule(static))
     for (int n = 0; n < npoin; ++n)
         mu_fluid[n] = mu_factor[n] * C1 * pow(Tem[n], 1.5) / (Tem[n] + S);
 }
@@ -924,6 +961,7 @@ void orc_lumped_solver_scal(int npoin, int npoin_w, const int *lpoin_w, const do
                             double *R)
 {
     (void)npoin;
+    ORC_PRAGMA(omp parallel for schedule(static))
     for (int i = 0; i < npoin_w; ++i) {
         int n = lpoin_w[i] - 1;
         R[n] = R[n] / Ml[n];
@@ -933,6 +971,7 @@ void orc_lumped_solver_scal(int npoin, int npoin_w, const int *lpoin_w, const do
 void orc_lumped_solver_vect(int npoin, int npoin_w, const int *lpoin_w, const double *Ml,
                             double *R)
 {
+    ORC_PRAGMA(omp parallel for schedule(static))
     for (int i = 0; i < npoin_w; ++i) {
         int n = lpoin_w[i] - 1;
         for (int d = 0; d < 3; ++d) R[n + (size_t)npoin * d] = R[n + (size_t)npoin * d] / Ml[n];
@@ -965,15 +1004,18 @@ void orc_smart_visc_spectral(int nelem, int npoin, int npoin_w, const int *conne
     memcpy(mue_l, mu_e, sizeof(double) * ne * 64);
     if (prm->flag_normalise_entropy == 1) {
         double maxEta_r = 0.0;
+        ORC_PRAGMA(omp parallel for schedule(static) reduction(+ : maxEta_r))
         for (int i = 0; i < npoin_w; ++i) maxEta_r = maxEta_r + eta[lpoin_w[i] - 1];
         double maxEta = maxEta_r / (double)npoin_w;
         double norm_r = 0.0;
+        ORC_PRAGMA(omp parallel for schedule(static) reduction(max : norm_r))
         for (int i = 0; i < npoin_w; ++i)
             norm_r = fmax_f(norm_r, fabs(eta[lpoin_w[i] - 1] - maxEta));
         norm = norm_r;
     } else {
         norm = 1.0;
     }
+    ORC_PRAGMA(omp parallel for schedule(static))
     for (int e = 0; e < nelem; ++e) {
         double maxJe = 0.0, minJe = 1000000.0, maxV = 0.0, maxC = 0.0;
         for (int g = 0; g < 64; ++g) {
@@ -1034,6 +1076,7 @@ double orc_adapt_dt_cfl(int nelem, int npoin, const int *connec, const double *h
 {
     const size_t np = (size_t)npoin, ne = (size_t)nelem;
     double dt_conv = 100000000000000.0, dt_diff = 100000000000000.0, dt_l = 100000000000000.0;
+    ORC_PRAGMA(omp parallel for schedule(static) reduction(min : dt_conv, dt_diff))
     for (int e = 0; e < nelem; ++e) {
         double L3 = 0.0;
         for (int a = 0; a < 64; ++a) {
@@ -1054,8 +1097,10 @@ double orc_adapt_dt_cfl(int nelem, int npoin, const int *connec, const double *h
         double hh = helem[e] / (double)(2.0 * ORC_PORDER + 1);
         double aux4 = cfl_diff * (hh * hh) / max_MU;
         dt_diff = fmin_f(dt_diff, aux4);
-        dt_l = fmin_f(dt_conv, dt_diff);
     }
+    /* the running minimum of the loop (dt_l = min(dt_conv, dt_diff) after
+     * every element, mod_time_ops.f90:54) ends at this value */
+    dt_l = fmin_f(dt_l, fmin_f(dt_conv, dt_diff));
     return dt_l;
 }
 
@@ -1110,6 +1155,7 @@ void orc_rk_4_main(int nelem, int npoin, int npoin_w, const int *connec, const i
 
     for (int istep = 0; istep < 4; ++istep) {
         /* time_integ.f90:239-248 */
+        ORC_PRAGMA(omp parallel for schedule(static))
         for (size_t n = 0; n < np; ++n) {
             aux_rho[n] = rho[n] - dt * a_i[istep] * Rmass[n];
             aux_E[n] = E[n] - dt * a_i[istep] * Rener[n];
@@ -1117,6 +1163,7 @@ void orc_rk_4_main(int nelem, int npoin, int npoin_w, const int *connec, const i
                 aux_q[n + np * d] = q[n + np * d] - dt * a_i[istep] * Rmom[n + np * d];
         }
         /* time_integ.f90:266-285 */
+        ORC_PRAGMA(omp parallel for schedule(static))
         for (int i = 0; i < npoin_w; ++i) {
             size_t n = (size_t)(lpoin_w[i] - 1);
             double umag = 0.0;
@@ -1153,16 +1200,19 @@ void orc_rk_4_main(int nelem, int npoin, int npoin_w, const int *connec, const i
                                 gmshAtoJ, gmshAtoK, aux_u, aux_q, aux_rho, aux_pr, aux_h, Rmass,
                                 Rmom, Rener);
         /* time_integ.f90:369-373 */
+        ORC_PRAGMA(omp parallel for schedule(static))
         for (size_t n = 0; n < np; ++n) {
             Rmass[n] = Rmass[n] + Rdiff_mass[n];
             Rener[n] = Rener[n] + Rdiff_ener[n];
         }
+        ORC_PRAGMA(omp parallel for schedule(static))
         for (size_t n = 0; n < 3 * np; ++n) Rmom[n] = Rmom[n] + Rdiff_mom[n];
         /* time_integ.f90:391-393 */
         orc_lumped_solver_scal(npoin, npoin_w, lpoin_w, Ml, Rmass);
         orc_lumped_solver_scal(npoin, npoin_w, lpoin_w, Ml, Rener);
         orc_lumped_solver_vect(npoin, npoin_w, lpoin_w, Ml, Rmom);
         /* time_integ.f90:399-409 */
+        ORC_PRAGMA(omp parallel for schedule(static))
         for (size_t n = 0; n < np; ++n) {
             Rmass_sum[n] = Rmass_sum[n] + b_i[istep] * Rmass[n];
             Rener_sum[n] = Rener_sum[n] + b_i[istep] * Rener[n];
@@ -1172,12 +1222,14 @@ void orc_rk_4_main(int nelem, int npoin, int npoin_w, const int *connec, const i
         }
     }
     /* time_integ.f90:417-426 */
+    ORC_PRAGMA(omp parallel for schedule(static))
     for (size_t n = 0; n < np; ++n) {
         rho[n] = rho[n] - dt * Rmass_sum[n];
         E[n] = E[n] - dt * Rener_sum[n];
         for (int d = 0; d < 3; ++d) q[n + np * d] = q[n + np * d] - dt * Rmom_sum[n + np * d];
     }
     /* time_integ.f90:445-463 */
+    ORC_PRAGMA(omp parallel for schedule(static))
     for (int i = 0; i < npoin_w; ++i) {
         size_t n = (size_t)(lpoin_w[i] - 1);
         double umag = 0.0;
@@ -1194,6 +1246,7 @@ void orc_rk_4_main(int nelem, int npoin, int npoin_w, const int *connec, const i
         eta[n] = (rho[n] / (gamma_gas - 1.0)) * log(pr[n] / pow(rho[n], gamma_gas));
     }
     /* time_integ.f90:467-471 */
+    ORC_PRAGMA(omp parallel for schedule(static))
     for (int i = 0; i < npoin_w; ++i) {
         size_t n = (size_t)(lpoin_w[i] - 1);
         Reta[n] = -Reta_sum[n];
