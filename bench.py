@@ -36,22 +36,49 @@ def kernel_ms(name):
     return tot.value, cnt.value
 
 
-def cpu_baseline(seconds: float):
-    """The oracle (single-threaded C restatement of the reference routines)
-    stepping the reference's CPU case: TGV on 8^3 HEX64 elements."""
+def _host_threads() -> int:
+    """Host threads the CPU baseline may use: the CPUs this process may run
+    on, capped by OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sol, m, gas, state, seconds: float):
+    """The oracle (the C restatement of the reference routines, built with
+    OpenMP: element and node loops on all host threads, atomic scatters)
+    stepping the same workload as the GPU -- TGV on the same mesh, with the
+    geometry the GPU run uses -- one rk_4_main + adapt_dt_cfl at a time, as
+    the reference's time loop does, until `seconds` have passed (at least
+    one step), timed per step like CFDSolverBase.f90:1552-1575."""
     import numpy as np
     import oracle
-    from sod2d_refmap_amd import cases, mesh
-    m = mesh.periodic_box(8)
-    gas = cases.GasParams()
-    s = cases.tgv_initial_state(m, gas)
+    from sod2d_refmap_amd.device import to_host
+    threads = _host_threads()
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    oracle.use_openmp(True)
     t = oracle.hex64_tables()
-    gpvol, He = oracle.elem_jacobian(m.connec_orig, m.coord, t)
-    Ml = oracle.lumped_mass_spectral(m.connec, gpvol, m.npoin)
-    helem = oracle.char_length(m.connec_orig, m.coord)
-    helem_l = oracle.char_length_spectral(m.connec_orig, m.coord)
+    ne, npn = m.nelem, m.npoin
+    He = to_host(sol.He, (3, 3, 64, ne))
+    gpvol = to_host(sol.gpvol, (1, 64, ne))
+    Ml = to_host(sol.Ml, (npn,))
+    helem = to_host(sol.helem, (ne,))
+    helem_l = to_host(sol.helem_l, (ne, 64))
+    s = {k: np.asfortranarray(v).copy(order="F") for k, v in state.items()}
     p = oracle.default_params(Rgas_inf=gas.Rgas)
-    zero = np.zeros((m.nelem, 64), order="F")
+    zero = np.zeros((ne, 64), order="F")
     steps = 0
     t0 = time.perf_counter()
     while True:
@@ -60,12 +87,15 @@ def cpu_baseline(seconds: float):
                          gas.Cp, gas.Prt, s["mu_factor"], s, p)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= seconds or steps >= 200:
+        log(f"cpu baseline: step {steps} at {el:.1f}s")
+        if el >= seconds:
             break
+    oracle.use_openmp(False)
     gdofs = 5.0 * m.npoin_w * 4 * steps / el / 1e9
-    return {"value": gdofs, "unit": "GDOF/s per RK4 stage", "cores": 1, "kind": "port",
-            "sample": f"oracle rk_4_main + adapt_dt_cfl, TGV 8^3 HEX64 ({m.npoin_w} nodes), {steps} steps "
-                      f"in {el:.1f} s on 1 host core"}
+    return {"value": gdofs, "unit": "GDOF/s per RK4 stage", "cores": threads, "kind": "port",
+            "sample": f"oracle rk_4_main + adapt_dt_cfl built with OpenMP, the same {m.n[0]}^3 TGV workload "
+                      f"({m.npoin_w} working nodes), {steps} steps in {el:.1f} s on {threads} host threads "
+                      f"({_cpu_model()})"}
 
 
 def measured_traffic(kernel: str, n: int):
@@ -180,10 +210,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64, help="elements per direction")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--batch", type=int, default=0, help="also time an ensemble of this many boxes")
+    ap.add_argument("--batch", type=int, default=48,
+                    help="also time the seeded ensemble of this many boxes (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="start with one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -211,7 +242,6 @@ def main():
     s = cases.tgv_initial_state(m, gas)
     log(f"mesh {n}^3: nelem={m.nelem} npoin_w={m.npoin_w} ({time.perf_counter() - t0:.1f}s)")
     sol = PeriodicSolver(m, gas, s, cfl_conv=0.5, cfl_diff=0.5)
-    del s
     dt = sol.adapt_dt()
     log(f"setup done, dt0={dt:.4e}")
     for _ in range(args.warmup):
@@ -314,7 +344,7 @@ def main():
     if args.batch > 0:
         out["batch"] = batch_speedup(args.batch, max(2, K // 2))
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(sol, m, gas, s, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

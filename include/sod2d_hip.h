@@ -350,29 +350,46 @@ int sod_time_loop_run_batched(sod_time_loop *loop, int nsteps, double cfl_conv, 
                               double *dt, double *time, int keep_levels);
 
 /* No reference counterpart: the configuration of the fused stage kernel used
- * by sod_rk_4_main and the time loops.
- *   mode 0 (default): 1 element per workgroup, residuals stored per element
- *     node and summed per node in a fixed order (bit-reproducible);
+ * by sod_rk_4_main and the time loops.  Assembly strategies:
+ *   0 element-node stores: each element's residuals are stored per element
+ *     node and the node update sums each node's contributions in a fixed
+ *     order (bit-reproducible);
+ *   1 atomic scatter into node arrays (the reference's assembly,
+ *     elem_convec.f90:442-455; its summation order varies from run to run);
+ *   2 colour-ordered scatter: elements are coloured from the connectivity so
+ *     that no two of one colour share a node, one launch per colour adds
+ *     into the node arrays without atomics (a fixed order; meshes where an
+ *     element holds a node twice, or that need more than 64 colours, cannot
+ *     use it).
+ * Modes:
+ *   mode 0 (default): 1 element per workgroup with element-node stores;
  *   mode 1: at the first step of each mesh the library times the
- *     configurations with element-node stores (1, 2 or 4 elements per
- *     workgroup) on that mesh and state, and keeps the fastest;
- *   mode 2: as 1, with device-scope atomic scatter into node arrays (the
- *     reference's assembly, elem_convec.f90:442-455; its summation order
- *     varies from run to run) among the candidates.
+ *     configurations with a fixed summation order (1, 2 or 4 elements per
+ *     workgroup with element-node stores, and colour-ordered scatter) on
+ *     that mesh and state, and keeps the fastest;
+ *   mode 2: as 1, with atomic scatter among the candidates.
  * Every configuration gives the reference's results to the stated
  * tolerances.  sod_full_rhs always uses the default. */
 int sod_set_autotune(int mode);
 /* No reference counterpart: pins one stage configuration (mode 3) instead of
- * a tuned one: elems_per_workgroup 1, 2 or 4 with atomic_scatter 0, or 4
- * with atomic_scatter 1.  sod_set_autotune returns to modes 0-2. */
-int sod_set_stage_config(int elems_per_workgroup, int atomic_scatter);
+ * a tuned one: elems_per_workgroup 1, 2 or 4 with assembly 0, 4 with
+ * assembly 1, or 1 with assembly 2.  sod_set_autotune returns to modes 0-2.
+ * A pinned colour-ordered configuration on a mesh that cannot be coloured
+ * makes the stepping call return SOD_ERR_UNSUPPORTED. */
+int sod_set_stage_config(int elems_per_workgroup, int assembly);
 /* The stage configuration of loop's mesh (loop NULL: the mesh most recently
- * stepped through sod_rk_4_main): elements per workgroup, atomic scatter
- * (0/1), the mode it was chosen under (0-2, or 3 when pinned) and the
+ * stepped through sod_rk_4_main): elements per workgroup, assembly (0-2 as
+ * above), the mode it was chosen under (0-2, or 3 when pinned) and the
  * measured time of one stage in ms (0 when not tuned).  Any output pointer
  * may be NULL. */
-int sod_stage_config(const sod_time_loop *loop, int *elems_per_workgroup, int *atomic_scatter, int *tuned_mode,
+int sod_stage_config(const sod_time_loop *loop, int *elems_per_workgroup, int *assembly, int *tuned_mode,
                      double *stage_ms);
+/* No reference counterpart: the element colouring of the colour-ordered
+ * assembly of loop's mesh (loop NULL: the mesh most recently stepped through
+ * sod_rk_4_main), once it has been built.  colour is a HOST array of nelem
+ * ints that receives each element's colour (0-based), ncolours the number
+ * of colours. */
+int sod_element_colouring(const sod_time_loop *loop, int *ncolours, int *colour, int nelem);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,7 @@ _SIGS = {
     "sod_time_loop_run_batched": [c_void_p, c_int, c_double, c_double, c_int, P, P, c_int],
     "sod_set_autotune": [c_int],
     "sod_set_stage_config": [c_int, c_int],
+    "sod_element_colouring": [c_void_p, ctypes.POINTER(c_int), P, c_int],
     "sod_stage_config": [c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                          ctypes.POINTER(c_double)],
 }
@@ -119,16 +120,25 @@ def call(name: str, *args) -> None:
     check(getattr(lib(), name)(*args))
 
 
+ASSEMBLY = {"pull": 0, "atomic": 1, "colour": 2}
+_ASSEMBLY_NAMES = {v: k for k, v in ASSEMBLY.items()}
+
+
 def set_autotune(mode: int) -> None:
     """How sod_rk_4_main and the time loops configure the stage kernel
-    (sod_set_autotune): 0 the default, 1 timed choice of the tile, 2 tile
-    and assembly strategy (atomic scatter among the candidates)."""
+    (sod_set_autotune): 0 the default, 1 timed choice among the
+    configurations with a fixed summation order (tiles with element-node
+    stores, colour-ordered scatter), 2 the same with atomic scatter among the
+    candidates."""
     call("sod_set_autotune", int(mode))
 
 
-def set_stage_config(elems_per_workgroup: int, atomic_scatter: bool) -> None:
-    """Pins one stage configuration (sod_set_stage_config)."""
-    call("sod_set_stage_config", int(elems_per_workgroup), int(bool(atomic_scatter)))
+def set_stage_config(elems_per_workgroup: int, assembly) -> None:
+    """Pins one stage configuration (sod_set_stage_config).  assembly:
+    "pull" (element-node stores), "atomic" or "colour", or the ABI's 0/1/2
+    (a bool keeps its meaning of atomic scatter or not)."""
+    a = ASSEMBLY[assembly] if isinstance(assembly, str) else int(assembly)
+    call("sod_set_stage_config", int(elems_per_workgroup), a)
 
 
 def stage_config(loop=None) -> dict:
@@ -136,8 +146,8 @@ def stage_config(loop=None) -> dict:
     mesh last stepped through sod_rk_4_main (sod_stage_config)."""
     nw, at, md, ms = c_int(), c_int(), c_int(), c_double()
     call("sod_stage_config", loop, ctypes.byref(nw), ctypes.byref(at), ctypes.byref(md), ctypes.byref(ms))
-    return {"elems_per_workgroup": nw.value, "atomic_scatter": bool(at.value), "mode": md.value,
-            "stage_ms": ms.value}
+    return {"elems_per_workgroup": nw.value, "assembly": _ASSEMBLY_NAMES.get(at.value, str(at.value)),
+            "atomic_scatter": at.value == 1, "mode": md.value, "stage_ms": ms.value}
 
 
 def exported_symbols() -> list[str]:

@@ -43,19 +43,25 @@ constexpr int BLK = 256;
 // sDt[o][d] = D[(o+d)&3][o] for the weak-form divergence.
 // Three wavefronts per SIMD (amdgpu_waves_per_eu(3)) measured faster than
 // the two the unconstrained register allocation gives.
-// NW elements (wavefronts) per workgroup.  ATOMIC: the residuals are added
-// into six node arrays (stride rstride) with device-scope atomics, as the
-// reference scatters them (elem_convec.f90:442-455, elem_diffu.f90:183-193);
-// otherwise they are stored per element node into relem.
+// NW elements (wavefronts) per workgroup.  ASM selects the assembly:
+//   ASM_PULL    the residuals are stored per element node into relem, and the
+//               node update sums each node's contributions in a fixed order;
+//   ASM_ATOMIC  they are added into six node arrays (stride rstride) with
+//               device-scope atomics, as the reference scatters them
+//               (elem_convec.f90:442-455, elem_diffu.f90:183-193);
+//   ASM_COLOR   one launch per element colour (no two elements of a colour
+//               share a node, no element holds a node twice): plain
+//               read-add-write into the six node arrays, in colour order.
+//               The workgroup's element is elist[blockIdx.x].
 // TOTAL_H: the convective energy residual of full_convec_ijk_H
 // (elem_convec.f90:20-208, selected by flag_total_enthalpy,
 // time_integ.f90:361-362) with the total enthalpy per volume H = E + p
 // instead of the h + k split of full_convec_ijk.
 // mufac is mu_factor with the Sutherland gate on (P.suth) and the caller's
 // mu_fluid otherwise.
-template <bool HAS_SGS, int NW, bool ATOMIC, bool TOTAL_H>
+template <bool HAS_SGS, int NW, int ASM, bool TOTAL_H>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_stage_rhs(
-    int nelem, const int* __restrict__ conn, const double* __restrict__ He,
+    int nelem, const int* __restrict__ elist, const int* __restrict__ conn, const double* __restrict__ He,
     const double* __restrict__ gpv, const double* __restrict__ mue, const double* __restrict__ musgs,
     const double* __restrict__ rho, const double* __restrict__ q0, const double* __restrict__ q1,
     const double* __restrict__ q2, const double* __restrict__ Ein, const double* __restrict__ mufac,
@@ -80,8 +86,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
     // elements, which share nodes, meet in the same L2; with one element per
     // workgroup the plain order measured 1.6% faster at 64^3.
     const int lb = NW == 1 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
-    const int e = lb * NW + w;
-    if (e >= nelem) return;   // uniform per wavefront; only wave-level sync below
+    if (lb * NW + w >= nelem) return;   // uniform per wavefront; only wave-level sync below
+    const int e = ASM == ASM_COLOR ? elist[blockIdx.x] : lb * NW + w;
     const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
     const size_t eb = (size_t)e * 64;
 
@@ -263,9 +269,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
 
     const double res[NRES] = {gv * Rm + nu_e * dD[0], gv * Rq[0] + dD[1], gv * Rq[1] + dD[2],
                               gv * Rq[2] + dD[3], gv * Re + dD[4] + kappa * dD[5], gv * Reta};
-    if (ATOMIC) {
+    if (ASM == ASM_ATOMIC) {
 #pragma unroll
         for (int f = 0; f < NRES; ++f) unsafeAtomicAdd(relem + f * rstride + n, res[f]);
+    } else if (ASM == ASM_COLOR) {
+        // this colour's elements own their nodes exclusively
+#pragma unroll
+        for (int f = 0; f < NRES; ++f) relem[f * rstride + n] += res[f];
     } else {
         // The residuals of this element at this node, stored in element-node
         // order; k_stage_update and k_assemble_nodes sum each node's
@@ -764,7 +774,12 @@ struct Prepared {
     // cfg_ms the measured time of one stage with it
     int cfg = 0, cfg_mode = 0;
     double cfg_ms = 0.0;
-    double* rn = nullptr;   // 6 x npoin node residuals of the atomic-scatter configuration
+    double* rn = nullptr;   // 6 x npoin node residuals of the atomic / colour-ordered configurations
+    // element colouring of the colour-ordered configuration (built on first
+    // use): elements colour by colour, and where each colour starts
+    int* color_elems = nullptr;
+    std::vector<int> color_off;
+    int color_state = 0;   // 0 not built, 1 built, 2 the mesh cannot be coloured
     void release() {
         (void)hipFree(conn);
         (void)hipFree(He);
@@ -780,6 +795,7 @@ struct Prepared {
         (void)hipFree(dscal);
         (void)hipFree(fpbuf);
         (void)hipFree(rn);
+        (void)hipFree(color_elems);
         *this = Prepared();
     }
 };
@@ -1069,26 +1085,97 @@ struct DtFold {
 
 // ---------------------------------------------------- stage configurations
 // The stage kernel's tile (elements = wavefronts per workgroup) and assembly
-// strategy (element-node stores summed per node in a fixed order, or
-// device-scope atomic scatter into node arrays, whose summation order varies
-// from run to run).  Entry 0 is the default (the fastest at 64^3 TGV: a
-// one-wavefront workgroup releases its LDS as soon as its element is done);
-// sod_set_autotune lets the library time the others on the caller's mesh and
-// keep the fastest.
+// strategy: element-node stores summed per node in a fixed order (pull),
+// device-scope atomic scatter into node arrays (its summation order varies
+// from run to run), or colour-ordered scatter (one launch per element
+// colour, plain read-add-write, a fixed order).  Entry 0 is the default (the
+// fastest at 64^3 TGV: a one-wavefront workgroup releases its LDS as soon as
+// its element is done); sod_set_autotune lets the library time the others on
+// the caller's mesh and keep the fastest.
 struct StageCfg {
     int nw;
-    bool atomic;
+    int assembly;   // ASM_PULL, ASM_ATOMIC, ASM_COLOR
 };
-constexpr StageCfg kStageCfgs[] = {{1, false}, {2, false}, {4, false}, {4, true}};
+constexpr StageCfg kStageCfgs[] = {{1, ASM_PULL}, {2, ASM_PULL}, {4, ASM_PULL}, {4, ASM_ATOMIC}, {1, ASM_COLOR}};
 constexpr int kNumStageCfgs = sizeof(kStageCfgs) / sizeof(kStageCfgs[0]);
 constexpr int kPinned = 16;   // autotune mode kPinned + c: configuration c pinned
 
-template <bool HS, int NW, bool AT, bool TH>
+// Element colouring for the colour-ordered assembly, from the lane-layout
+// connectivity alone: elements in increasing order take the lowest colour
+// that no element sharing one of their nodes holds (greedy; at most 64
+// colours).  Returns false, leaving P without a colouring, when an element
+// holds one node twice (its lanes would race) or 64 colours do not suffice.
+// Elements are listed colour by colour, increasing within a colour.
+static bool build_colouring(Prepared& P, const std::vector<int>& cl, int nelem, int npoin,
+                            std::vector<int>& elist, std::vector<int>& off) {
+    std::vector<unsigned long long> used((size_t)npoin, 0ULL);
+    std::vector<int> stamp((size_t)npoin, -1);
+    std::vector<unsigned char> col((size_t)nelem, 0);
+    for (int e = 0; e < nelem; ++e) {
+        unsigned long long m = 0;
+        const int* c = cl.data() + (size_t)e * 64;
+        for (int l = 0; l < 64; ++l) {
+            if (stamp[c[l]] == e) return false;   // a node twice in one element
+            stamp[c[l]] = e;
+            m |= used[c[l]];
+        }
+        if (m == ~0ULL) return false;
+        const int k = __builtin_ctzll(~m);
+        for (int l = 0; l < 64; ++l) used[c[l]] |= 1ULL << k;
+        col[e] = (unsigned char)k;
+    }
+    int nc = 0;
+    for (int e = 0; e < nelem; ++e) nc = std::max(nc, (int)col[e] + 1);
+    off.assign((size_t)nc + 1, 0);
+    for (int e = 0; e < nelem; ++e) ++off[(size_t)col[e] + 1];
+    for (int k = 0; k < nc; ++k) off[k + 1] += off[k];
+    elist.resize((size_t)nelem);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int e = 0; e < nelem; ++e) elist[fill[col[e]]++] = e;
+    (void)P;
+    return true;
+}
+
+// Builds P's colouring on first use (host work over the lane-layout
+// connectivity); SOD_ERR_UNSUPPORTED when the mesh cannot be coloured.
+static int ensure_colouring(Prepared& P) {
+    if (P.color_state == 1) return SOD_OK;
+    if (P.color_state == 2)
+        return fail(SOD_ERR_UNSUPPORTED, "colour-ordered assembly: an element holds a node twice, or more than "
+                                         "64 colours would be needed");
+    const size_t nl = (size_t)P.nelem * 64;
+    std::vector<int> cl(nl), elist, off;
+    SOD_HIP(hipStreamSynchronize(rt().stream));
+    if (nl > 0) SOD_HIP(hipMemcpy(cl.data(), P.conn, sizeof(int) * nl, hipMemcpyDeviceToHost));
+    if (!build_colouring(P, cl, P.nelem, P.npoin, elist, off)) {
+        P.color_state = 2;
+        return ensure_colouring(P);
+    }
+    SOD_HIP(hipMalloc(&P.color_elems, sizeof(int) * (elist.size() + 1)));
+    if (!elist.empty())
+        SOD_HIP(hipMemcpy(P.color_elems, elist.data(), sizeof(int) * elist.size(), hipMemcpyHostToDevice));
+    P.color_off = off;
+    P.color_state = 1;
+    return SOD_OK;
+}
+
+template <bool HS, int NW, int AS, bool TH>
 static void launch_rhs_t(hipStream_t s, int nelem, const Prepared& P, const double* const in[6], long np,
                          const StageParams& sp) {
-    hipLaunchKernelGGL((k_stage_rhs<HS, NW, AT, TH>), dim3(grid_for(nelem, NW)), dim3(64 * NW), 0, s, nelem,
-                       P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2], in[3], in[4], in[5],
-                       AT ? P.rn : P.relem, np, sp);
+    if (AS == ASM_COLOR) {
+        const int nc = (int)P.color_off.size() - 1;
+        for (int k = 0; k < nc; ++k) {
+            const int b = P.color_off[k], cnt = P.color_off[k + 1] - b;
+            if (cnt > 0)
+                hipLaunchKernelGGL((k_stage_rhs<HS, NW, AS, TH>), dim3(cnt), dim3(64 * NW), 0, s, cnt,
+                                   P.color_elems + b, P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2],
+                                   in[3], in[4], in[5], P.rn, np, sp);
+        }
+        return;
+    }
+    hipLaunchKernelGGL((k_stage_rhs<HS, NW, AS, TH>), dim3(grid_for(nelem, NW)), dim3(64 * NW), 0, s, nelem,
+                       (const int*)nullptr, P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2], in[3], in[4],
+                       in[5], AS == ASM_PULL ? P.relem : P.rn, np, sp);
 }
 
 // stage kernel of configuration `cfg` on the state in[] = rho, q0, q1, q2, E
@@ -1098,10 +1185,11 @@ template <bool HS, bool TH>
 static void launch_rhs_hs(int cfg, hipStream_t s, int nelem, const Prepared& P, const double* const in[6], long np,
                           const StageParams& sp) {
     switch (cfg) {
-        case 1: launch_rhs_t<HS, 2, false, TH>(s, nelem, P, in, np, sp); break;
-        case 2: launch_rhs_t<HS, 4, false, TH>(s, nelem, P, in, np, sp); break;
-        case 3: launch_rhs_t<HS, 4, true, TH>(s, nelem, P, in, np, sp); break;
-        default: launch_rhs_t<HS, 1, false, TH>(s, nelem, P, in, np, sp); break;
+        case 1: launch_rhs_t<HS, 2, ASM_PULL, TH>(s, nelem, P, in, np, sp); break;
+        case 2: launch_rhs_t<HS, 4, ASM_PULL, TH>(s, nelem, P, in, np, sp); break;
+        case 3: launch_rhs_t<HS, 4, ASM_ATOMIC, TH>(s, nelem, P, in, np, sp); break;
+        case 4: launch_rhs_t<HS, 1, ASM_COLOR, TH>(s, nelem, P, in, np, sp); break;
+        default: launch_rhs_t<HS, 1, ASM_PULL, TH>(s, nelem, P, in, np, sp); break;
     }
 }
 // total_h: full_convec_ijk_H in place of full_convec_ijk (flag_total_enthalpy)
@@ -1213,7 +1301,8 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
         }
         SOD_PROF("stage_update");
         const dim3 grid(stage == 3 ? ugrid : node_grid(npoin_w));
-        launch_update(stage == 3, B != nullptr, kStageCfgs[P.cfg].atomic, grid, s, npoin, npoin_w, a.lpoin_w, stage,
+        launch_update(stage == 3, B != nullptr, kStageCfgs[P.cfg].assembly != ASM_PULL, grid, s, npoin, npoin_w,
+                      a.lpoin_w, stage,
                       d_dt, a.gamma_gas, a.Rgas, st, B ? B->node_inst : nullptr);
     }
     if (P.nslave > 0 && st.suth)
@@ -1245,15 +1334,27 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
     }
 }
 
-static_assert(kNumStageCfgs == 4 && kStageCfgs[0].nw == 1 && !kStageCfgs[0].atomic && kStageCfgs[1].nw == 2 &&
-                  !kStageCfgs[1].atomic && kStageCfgs[2].nw == 4 && !kStageCfgs[2].atomic &&
-                  kStageCfgs[3].nw == 4 && kStageCfgs[3].atomic,
+static_assert(kNumStageCfgs == 5 && kStageCfgs[0].nw == 1 && kStageCfgs[0].assembly == ASM_PULL &&
+                  kStageCfgs[1].nw == 2 && kStageCfgs[1].assembly == ASM_PULL && kStageCfgs[2].nw == 4 &&
+                  kStageCfgs[2].assembly == ASM_PULL && kStageCfgs[3].nw == 4 &&
+                  kStageCfgs[3].assembly == ASM_ATOMIC && kStageCfgs[4].nw == 1 &&
+                  kStageCfgs[4].assembly == ASM_COLOR,
               "launch_rhs_hs dispatches on these entries");
 
+// Node residual arrays of the scatter configurations (atomic and
+// colour-ordered), zeroed once; the node update re-zeroes what it reads.
+static int ensure_node_residuals(Prepared& P, size_t np, hipStream_t s) {
+    if (P.rn != nullptr) return SOD_OK;
+    SOD_HIP(hipMalloc(&P.rn, sizeof(double) * NRES * np + 8));
+    SOD_HIP(hipMemsetAsync(P.rn, 0, sizeof(double) * NRES * np, s));
+    return SOD_OK;
+}
+
 // Sets P.cfg for the autotune mode (sod_set_autotune): 0 keeps the default
-// configuration; 1 takes the fastest configuration with element-node stores;
-// 2 the fastest of all, atomic scatter included; kPinned + c pins entry c
-// (sod_set_stage_config).  Each candidate runs the
+// configuration; 1 takes the fastest configuration whose summation order is
+// fixed (element-node stores, or colour-ordered scatter when the mesh can be
+// coloured); 2 the fastest of all, atomic scatter included; kPinned + c pins
+// entry c (sod_set_stage_config).  Each candidate runs the
 // first stage of a step on the caller's level-2 state -- the stage kernel and
 // the node update that sums its residuals -- once to warm up and three times
 // timed with HIP events on the library stream; the smallest time wins.  Only
@@ -1267,20 +1368,17 @@ static int choose_stage_cfg(Prepared& P, const FusedArgs& a, bool has_sgs, const
     const size_t np = (size_t)a.npoin;
     if (mode == 0 || mode >= kPinned) {
         const int c = mode == 0 ? 0 : mode - kPinned;
-        if (kStageCfgs[c].atomic && P.rn == nullptr) {
-            SOD_HIP(hipMalloc(&P.rn, sizeof(double) * NRES * np + 8));
-            SOD_HIP(hipMemsetAsync(P.rn, 0, sizeof(double) * NRES * np, s));
-        }
+        if (kStageCfgs[c].assembly == ASM_COLOR) SOD_TRY(ensure_colouring(P));
+        if (kStageCfgs[c].assembly != ASM_PULL) SOD_TRY(ensure_node_residuals(P, np, s));
         P.cfg = c;
         P.cfg_mode = mode;
         P.cfg_ms = 0.0;
         return SOD_OK;
     }
     const bool try_atomic = mode >= 2;
-    if (try_atomic && P.rn == nullptr) {
-        SOD_HIP(hipMalloc(&P.rn, sizeof(double) * NRES * np + 8));
-        SOD_HIP(hipMemsetAsync(P.rn, 0, sizeof(double) * NRES * np, s));
-    }
+    // a mesh that cannot be coloured just drops that candidate
+    const bool try_color = ensure_colouring(P) == SOD_OK;
+    SOD_TRY(ensure_node_residuals(P, np, s));
     const StepState st = step_state(P, a);
     const sod_numerical_params& prm = rt().params;
     const StageParams sp = make_stage_params(a.gamma_gas, a.Rgas, a.Cp, a.Prt, prm);
@@ -1291,13 +1389,15 @@ static int choose_stage_cfg(Prepared& P, const FusedArgs& a, bool has_sgs, const
     double best = HUGE_VAL;
     int best_cfg = 0;
     for (int c = 0; c < kNumStageCfgs && err == hipSuccess; ++c) {
-        if (kStageCfgs[c].atomic && !try_atomic) continue;
+        if (kStageCfgs[c].assembly == ASM_ATOMIC && !try_atomic) continue;
+        if (kStageCfgs[c].assembly == ASM_COLOR && !try_color) continue;
         double tmin = HUGE_VAL;
         for (int rep = 0; rep < 4 && err == hipSuccess; ++rep) {
             err = hipEventRecord(e0, s);
             if (a.nelem > 0)
                 launch_stage_rhs(c, has_sgs, prm.flag_total_enthalpy != 0, s, a.nelem, P, in, (long)np, sp);
-            launch_update(false, B != nullptr, kStageCfgs[c].atomic, dim3(node_grid(a.npoin_w)), s, a.npoin,
+            launch_update(false, B != nullptr, kStageCfgs[c].assembly != ASM_PULL, dim3(node_grid(a.npoin_w)), s,
+                          a.npoin,
                           a.npoin_w, a.lpoin_w,
                           0, d_dt, a.gamma_gas, a.Rgas, st, B ? B->node_inst : nullptr);
             if (err == hipSuccess) err = hipGetLastError();
@@ -1318,7 +1418,7 @@ static int choose_stage_cfg(Prepared& P, const FusedArgs& a, bool has_sgs, const
     P.cfg = best_cfg;
     P.cfg_mode = mode;
     P.cfg_ms = best;
-    if (!kStageCfgs[best_cfg].atomic && P.rn != nullptr) {
+    if (kStageCfgs[best_cfg].assembly == ASM_PULL && P.rn != nullptr) {
         SOD_HIP(hipStreamSynchronize(s));
         (void)hipFree(P.rn);
         P.rn = nullptr;
@@ -1885,23 +1985,26 @@ extern "C" int sod_time_loop_destroy(sod_time_loop* L) {
 
 extern "C" int sod_set_autotune(int mode) {
     if (mode < 0 || mode > 2)
-        return fail(SOD_ERR_ARG, "sod_set_autotune: mode %d (0: default, 1: tiles, 2: tiles and atomic scatter)",
+        return fail(SOD_ERR_ARG,
+                    "sod_set_autotune: mode %d (0: default, 1: fixed-order configurations, 2: all, atomic "
+                    "scatter included)",
                     mode);
     rt().autotune = mode;
     return SOD_OK;
 }
 
-extern "C" int sod_set_stage_config(int elems_per_workgroup, int atomic_scatter) {
+extern "C" int sod_set_stage_config(int elems_per_workgroup, int assembly) {
+    static const char* names[3] = {"element-node stores", "atomic scatter", "colour-ordered scatter"};
     for (int c = 0; c < kNumStageCfgs; ++c)
-        if (kStageCfgs[c].nw == elems_per_workgroup && kStageCfgs[c].atomic == (atomic_scatter != 0)) {
+        if (kStageCfgs[c].nw == elems_per_workgroup && kStageCfgs[c].assembly == assembly) {
             rt().autotune = kPinned + c;
             return SOD_OK;
         }
     return fail(SOD_ERR_ARG, "sod_set_stage_config: no configuration with %d elements per workgroup and %s",
-                elems_per_workgroup, atomic_scatter ? "atomic scatter" : "element-node stores");
+                elems_per_workgroup, assembly >= 0 && assembly <= 2 ? names[assembly] : "that assembly");
 }
 
-extern "C" int sod_stage_config(const sod_time_loop* loop, int* elems_per_workgroup, int* atomic_scatter,
+extern "C" int sod_stage_config(const sod_time_loop* loop, int* elems_per_workgroup, int* assembly,
                                 int* tuned_mode, double* stage_ms) {
     const Prepared* P = nullptr;
     if (loop != nullptr) {
@@ -1915,8 +2018,33 @@ extern "C" int sod_stage_config(const sod_time_loop* loop, int* elems_per_workgr
     if (P == nullptr) return fail(SOD_ERR_ARG, "sod_stage_config: no mesh has been stepped yet");
     const StageCfg& c = kStageCfgs[P->cfg];
     if (elems_per_workgroup) *elems_per_workgroup = c.nw;
-    if (atomic_scatter) *atomic_scatter = c.atomic ? 1 : 0;
+    if (assembly) *assembly = c.assembly;
     if (tuned_mode) *tuned_mode = P->cfg_mode >= kPinned ? 3 : P->cfg_mode;
     if (stage_ms) *stage_ms = P->cfg_ms;
+    return SOD_OK;
+}
+
+extern "C" int sod_element_colouring(const sod_time_loop* loop, int* ncolours, int* colour, int nelem) {
+    const Prepared* P = nullptr;
+    if (loop != nullptr) {
+        P = &loop->P;
+    } else {
+        const PrepSlot* last = nullptr;
+        for (const auto& sl : g_slots)
+            if (sl.P.nelem >= 0 && (last == nullptr || sl.used > last->used)) last = &sl;
+        if (last != nullptr) P = &last->P;
+    }
+    if (P == nullptr || P->color_state != 1)
+        return fail(SOD_ERR_ARG, "sod_element_colouring: no coloured mesh (step one with colour-ordered assembly)");
+    if (nelem != P->nelem || colour == nullptr || ncolours == nullptr)
+        return fail(SOD_ERR_ARG, "sod_element_colouring: nelem %d does not match the mesh (%d) or NULL output",
+                    nelem, P->nelem);
+    const int nc = (int)P->color_off.size() - 1;
+    std::vector<int> elist((size_t)nelem);
+    SOD_HIP(hipStreamSynchronize(rt().stream));
+    if (nelem > 0) SOD_HIP(hipMemcpy(elist.data(), P->color_elems, sizeof(int) * nelem, hipMemcpyDeviceToHost));
+    for (int k = 0; k < nc; ++k)
+        for (int t = P->color_off[k]; t < P->color_off[k + 1]; ++t) colour[elist[t]] = k;
+    *ncolours = nc;
     return SOD_OK;
 }

@@ -97,6 +97,8 @@ inline StageParams make_stage_params(double gamma, double Rgas, double Cp, doubl
 }
 
 enum { P_RHO, P_U0, P_U1, P_U2, P_P, P_H, P_K, P_ETA, P_T, NPRIM };
+// assembly of the stage kernel's residuals (k_stage_rhs)
+enum { ASM_PULL = 0, ASM_ATOMIC = 1, ASM_COLOR = 2 };
 enum { R_MASS, R_MOM0, R_MOM1, R_MOM2, R_ENER, R_ETA, NRES };
 
 // Lane-layout view of the mesh prepared for the drop-in entry points

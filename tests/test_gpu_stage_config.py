@@ -1,10 +1,12 @@
 """Stage-kernel configurations (sod_set_autotune, sod_set_stage_config): 1, 2
-or 4 elements per workgroup with element-node stores, and 4 with atomic
-scatter into node arrays.  Every configuration steps a curvilinear mesh with
-permuted elements to the oracle's tolerances; the autotuner picks one of the
+or 4 elements per workgroup with element-node stores, 4 with atomic scatter
+into node arrays, and colour-ordered scatter (one launch per element colour,
+no atomics).  Every configuration steps a curvilinear mesh with permuted
+elements to the oracle's tolerances; the autotuner picks one of the
 candidates of its mode and reports it; the device time loop and a batched
 ensemble run with a pinned or tuned configuration and match the default
-eager steps.  Every test returns the library to the default configuration."""
+eager steps; the colour-ordered steps are bit-reproducible.  Every test
+returns the library to the default configuration."""
 import numpy as np
 import pytest
 
@@ -16,7 +18,7 @@ from sod2d_refmap_amd.device import numerical_params, set_numerical_params
 pytestmark = pytest.mark.gpu
 
 T = oracle.hex64_tables()
-CFGS = [(4, False), (2, False), (1, False), (4, True)]
+CFGS = [(4, "pull"), (2, "pull"), (1, "pull"), (4, "atomic"), (1, "colour")]
 KEYS = ("rho", "q", "E", "u", "pr", "Tem", "eta", "csound", "machno", "mu_fluid", "mu_e", "mue_l")
 
 
@@ -45,17 +47,17 @@ def _oracle_steps(m, g, gas, s, dt, nsteps):
     return out
 
 
-@pytest.mark.parametrize("nw,atomic", CFGS, ids=[f"nw{n}{'_atomic' if a else ''}" for n, a in CFGS])
-def test_each_configuration_matches_oracle(gpu, default_after, nw, atomic):
+@pytest.mark.parametrize("nw,assembly", CFGS, ids=[f"nw{n}_{a}" for n, a in CFGS])
+def test_each_configuration_matches_oracle(gpu, default_after, nw, assembly):
     from sod2d_refmap_amd.solver import PeriodicSolver
     m, gas, s = _case()
     g = mesh_geometry(m)
     dt = oracle.adapt_dt_cfl(m.connec, g["helem"], s["u"], s["csound"], 0.5, 0.5, s["mu_fluid"],
                              np.zeros((m.nelem, 64), order="F"), s["rho"])
     ref = _oracle_steps(m, g, gas, s, dt, 10)
-    _lib.set_stage_config(nw, atomic)
+    _lib.set_stage_config(nw, assembly)
     sol = PeriodicSolver(m, gas, s)
-    print(f"[stage config] {nw} elements per workgroup, atomic={atomic}", flush=True)
+    print(f"[stage config] {nw} elements per workgroup, assembly={assembly}", flush=True)
     for step in range(10):
         sol.rk_4_main(dt)
         if step in (0, 9):
@@ -65,13 +67,15 @@ def test_each_configuration_matches_oracle(gpu, default_after, nw, atomic):
                 err = rel_l2(out[k], ref[step][k])
                 assert err < tol, (step, k, err)
     c = _lib.stage_config()
-    assert (c["elems_per_workgroup"], c["atomic_scatter"], c["mode"]) == (nw, atomic, 3)
+    assert (c["elems_per_workgroup"], c["assembly"], c["mode"]) == (nw, assembly, 3)
     sol.close()
 
 
 def test_unknown_configuration_is_rejected(gpu, default_after):
     with pytest.raises(_lib.Sod2dError, match="no configuration"):
-        _lib.set_stage_config(8, False)
+        _lib.set_stage_config(8, "pull")
+    with pytest.raises(_lib.Sod2dError, match="no configuration"):
+        _lib.set_stage_config(4, "colour")
     with pytest.raises(_lib.Sod2dError, match="mode"):
         _lib.set_autotune(3)
 
@@ -91,7 +95,8 @@ def test_autotuned_step_matches_oracle(gpu, default_after, mode):
     c = _lib.stage_config()
     print(f"[autotune {mode}] chose {c}", flush=True)
     assert c["mode"] == mode and c["stage_ms"] > 0.0
-    assert (c["elems_per_workgroup"], c["atomic_scatter"]) in (CFGS if mode == 2 else CFGS[:3])
+    fixed_order = [x for x in CFGS if x[1] != "atomic"]
+    assert (c["elems_per_workgroup"], c["assembly"]) in (CFGS if mode == 2 else fixed_order)
     out = sol.state()
     for k in KEYS:
         err = rel_l2(out[k], ref[1][k])
@@ -102,18 +107,19 @@ def test_autotuned_step_matches_oracle(gpu, default_after, mode):
 def test_time_loop_across_configurations(gpu, default_after):
     """The loop re-captures its step when the configuration changes between
     runs: 2 steps pinned to 1 element per workgroup, 2 with atomic scatter,
-    2 tuned with every candidate, against 6 default eager steps."""
+    2 colour-ordered, 2 tuned with every candidate, against 8 default eager
+    steps."""
     from sod2d_refmap_amd.solver import PeriodicSolver
     m, gas, s = _case(perm_seed=13)
     A = PeriodicSolver(m, gas, s)
     B = PeriodicSolver(m, gas, s)
     A.adapt_dt()
     B.adapt_dt()
-    for _ in range(6):
+    for _ in range(8):
         A.rk_4_main(A.dt)
         A.time += A.dt
         A.adapt_dt()
-    for setting in ((1, False), (4, True), "tuned"):
+    for setting in ((1, "pull"), (4, "atomic"), (1, "colour"), "tuned"):
         print(f"[time loop] configuration {setting}", flush=True)
         if setting == "tuned":
             _lib.set_autotune(2)
@@ -122,7 +128,7 @@ def test_time_loop_across_configurations(gpu, default_after):
         B.run_device(2, keep_levels=False)
         c = _lib.stage_config(B._loop)
         if setting != "tuned":
-            assert (c["elems_per_workgroup"], c["atomic_scatter"]) == setting
+            assert (c["elems_per_workgroup"], c["assembly"]) == setting
         else:
             assert c["mode"] == 2 and c["stage_ms"] > 0.0
     _lib.set_autotune(0)
@@ -144,11 +150,15 @@ def test_ensemble_with_atomic_scatter_and_tuned(gpu, default_after):
         members.append((m, gas, cases.tgv_initial_state(m, gas, perturb_seed=seed, perturb_amp=1e-2)))
     D = Ensemble(members)
     D.run(3)
-    _lib.set_stage_config(4, True)
+    _lib.set_stage_config(4, "atomic")
     E = Ensemble(members)
     E.run(3)
     c = E.stage_config()
-    assert (c["elems_per_workgroup"], c["atomic_scatter"]) == (4, True)
+    assert (c["elems_per_workgroup"], c["assembly"]) == (4, "atomic")
+    _lib.set_stage_config(1, "colour")
+    G = Ensemble(members)
+    G.run(3)
+    assert G.stage_config()["assembly"] == "colour"
     # the tuner on a batched loop (per-instance dt in its trial stages)
     _lib.set_autotune(2)
     F = Ensemble(members)
@@ -156,7 +166,7 @@ def test_ensemble_with_atomic_scatter_and_tuned(gpu, default_after):
     cf = F.stage_config()
     print(f"[stage config] tuned ensemble: {cf}", flush=True)
     assert cf["mode"] == 2 and cf["stage_ms"] > 0.0
-    for X in (E, F):
+    for X in (E, F, G):
         for i in range(len(members)):
             got, ref = X.member_state(i), D.member_state(i)
             for k in KEYS:
@@ -165,3 +175,60 @@ def test_ensemble_with_atomic_scatter_and_tuned(gpu, default_after):
     D.close()
     E.close()
     F.close()
+    G.close()
+
+
+def _colouring(m):
+    """The greedy element colouring the library builds, restated: elements
+    in increasing order take the lowest colour no element sharing a node
+    holds (from the connectivity only)."""
+    conn = np.asarray(m.connec) - 1
+    used = {}
+    col = np.zeros(m.nelem, dtype=np.int64)
+    for e in range(m.nelem):
+        taken = set()
+        for n in conn[e]:
+            taken |= used.get(int(n), set())
+        k = 0
+        while k in taken:
+            k += 1
+        col[e] = k
+        for n in conn[e]:
+            used.setdefault(int(n), set()).add(k)
+    return col
+
+
+@pytest.mark.parametrize("kw", [dict(n=4), dict(n=3, curved_amp=0.05, perm_seed=11),
+                                dict(n=(3, 2, 4), L=(2 * np.pi, 4 * np.pi, 2 * np.pi), perm_seed=3)])
+def test_colouring_is_valid(gpu, default_after, kw):
+    """No two elements of one colour share a node, bit for bit, on the
+    colouring the library uses (read back through sod_element_colouring);
+    it equals the greedy restatement; two colour-ordered runs from the same
+    state are bit-identical."""
+    import ctypes
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    m = pmesh.periodic_box(**kw)
+    gas = cases.GasParams()
+    s = cases.tgv_initial_state(m, gas, perturb_seed=2, perturb_amp=2e-2)
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    _lib.set_stage_config(1, "colour")
+    outs = []
+    for rep in range(2):
+        sol = PeriodicSolver(m, gas, s)
+        dt = sol.adapt_dt()
+        for _ in range(3):
+            sol.rk_4_main(dt)
+        outs.append(sol.state())
+        if rep == 0:
+            col = np.zeros(m.nelem, dtype=np.int32)
+            nc = ctypes.c_int(0)
+            _lib.call("sod_element_colouring", None, ctypes.byref(nc), col.ctypes.data, m.nelem)
+        sol.close()
+    conn = np.asarray(m.connec) - 1
+    for k in range(nc.value):
+        nodes = conn[col == k].ravel()
+        assert nodes.size == np.unique(nodes).size, k
+    assert np.array_equal(col, _colouring(m))
+    print(f"[colouring] {m.nelem} elements, {nc.value} colours", flush=True)
+    for k in KEYS:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
