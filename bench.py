@@ -215,6 +215,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batch", type=int, default=48,
                     help="also time the seeded ensemble of this many boxes (0: skip)")
+    ap.add_argument("--stage-config", default="",
+                    help="pin the stage kernel configuration as ELEMS_PER_WORKGROUP,ASSEMBLY "
+                         "(assembly pull|atomic|colour; 0,pull: persistent wavefronts)")
+    ap.add_argument("--autotune", type=int, default=0, help="stage configuration mode (sod_set_autotune)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="start with one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -229,6 +233,13 @@ def main():
     from sod2d_refmap_amd.solver import PeriodicSolver
 
     n = args.n
+    from sod2d_refmap_amd.device import init as dev_init
+    dev_init(0)
+    if args.stage_config:
+        nw, asm = args.stage_config.split(",")
+        _lib.set_stage_config(int(nw), asm)
+    elif args.autotune:
+        _lib.set_autotune(args.autotune)
     if args.copy_probe:
         x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)

@@ -365,15 +365,18 @@ int sod_time_loop_run_batched(sod_time_loop *loop, int nsteps, double cfl_conv, 
  *   mode 0 (default): 1 element per workgroup with element-node stores;
  *   mode 1: at the first step of each mesh the library times the
  *     configurations with a fixed summation order (1, 2 or 4 elements per
- *     workgroup with element-node stores, and colour-ordered scatter) on
- *     that mesh and state, and keeps the fastest;
+ *     workgroup or persistent wavefronts with element-node stores, and
+ *     colour-ordered scatter) on that mesh and state, and keeps the fastest;
  *   mode 2: as 1, with atomic scatter among the candidates.
  * Every configuration gives the reference's results to the stated
  * tolerances.  sod_full_rhs always uses the default. */
 int sod_set_autotune(int mode);
 /* No reference counterpart: pins one stage configuration (mode 3) instead of
  * a tuned one: elems_per_workgroup 1, 2 or 4 with assembly 0, 4 with
- * assembly 1, or 1 with assembly 2.  sod_set_autotune returns to modes 0-2.
+ * assembly 1, or 1 with assembly 2; elems_per_workgroup 0 with assembly 0
+ * selects persistent one-wavefront workgroups (three per SIMD, each taking
+ * every G-th element and loading the next element's node ids while it
+ * works on the current one).  sod_set_autotune returns to modes 0-2.
  * A pinned colour-ordered configuration on a mesh that cannot be coloured
  * makes the stepping call return SOD_ERR_UNSUPPORTED. */
 int sod_set_stage_config(int elems_per_workgroup, int assembly);

@@ -59,7 +59,12 @@ constexpr int BLK = 256;
 // instead of the h + k split of full_convec_ijk.
 // mufac is mu_factor with the Sutherland gate on (P.suth) and the caller's
 // mu_fluid otherwise.
-template <bool HAS_SGS, int NW, int ASM, bool TOTAL_H>
+// PERSIST (one wavefront per workgroup, element-node stores): the wavefront
+// of workgroup b takes elements b, b + G, b + 2G, ... (G = grid size) and
+// loads the next element's node ids after the current element's own loads,
+// so that an element waits for one round of loads (its gathers, issued with
+// its geometry) instead of two (the node ids, then the gathers).
+template <bool HAS_SGS, int NW, int ASM, bool TOTAL_H, bool PERSIST = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_stage_rhs(
     int nelem, const int* __restrict__ elist, const int* __restrict__ conn, const double* __restrict__ He,
     const double* __restrict__ gpv, const double* __restrict__ mue, const double* __restrict__ musgs,
@@ -87,12 +92,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
     // workgroup the plain order measured 1.6% faster at 64^3.
     const int lb = NW == 1 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     if (lb * NW + w >= nelem) return;   // uniform per wavefront; only wave-level sync below
-    const int e = ASM == ASM_COLOR ? elist[blockIdx.x] : lb * NW + w;
     const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
+    int e = ASM == ASM_COLOR ? elist[blockIdx.x] : lb * NW + w;
+    int n = conn[(size_t)e * 64 + l];
+    for (;;) {   // one element, or (PERSIST) every G-th element
     const size_t eb = (size_t)e * 64;
 
     // ---- gather + equation of state (time_integ.f90:266-285) -----------
-    const int n = conn[eb + l];
     const double r = rho[n];
     const double m0 = q0[n], m1 = q1[n], m2 = q2[n], en = Ein[n], mf = mufac[n];
     const double ri = 1.0 / r;
@@ -117,6 +123,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
     const double gv = gpv[eb + l];
     const double mu_e = mue[eb + l];
     const double mu_s = HAS_SGS ? musgs[eb + l] : 0.0;
+    // the next element's node ids, issued after this element's loads so
+    // that waiting for those does not wait for them
+    const int e_next = e + (int)gridDim.x;
+    int n_next = 0;
+    if (PERSIST && e_next < nelem) n_next = conn[(size_t)e_next * 64 + l];
 
     double(*S)[64] = sh[w];
     const int me = swz(i, j, k);
@@ -284,6 +295,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
         o[0] = make_double2(res[0], res[1]);
         o[1] = make_double2(res[2], res[3]);
         o[2] = make_double2(res[4], res[5]);
+    }
+    if (!PERSIST || e_next >= nelem) break;
+    e = e_next;
+    n = n_next;
+    wave_sync();   // this element's LDS reads before the next element's writes
     }
 }
 
@@ -1092,11 +1108,14 @@ struct DtFold {
 // fastest at 64^3 TGV: a one-wavefront workgroup releases its LDS as soon as
 // its element is done); sod_set_autotune lets the library time the others on
 // the caller's mesh and keep the fastest.
+// nw == 0: the persistent one-wavefront kernel (k_stage_rhs<..., PERSIST>)
+// with element-node stores.
 struct StageCfg {
     int nw;
     int assembly;   // ASM_PULL, ASM_ATOMIC, ASM_COLOR
 };
-constexpr StageCfg kStageCfgs[] = {{1, ASM_PULL}, {2, ASM_PULL}, {4, ASM_PULL}, {4, ASM_ATOMIC}, {1, ASM_COLOR}};
+constexpr StageCfg kStageCfgs[] = {{1, ASM_PULL}, {2, ASM_PULL}, {4, ASM_PULL}, {4, ASM_ATOMIC}, {1, ASM_COLOR},
+                                   {0, ASM_PULL}};
 constexpr int kNumStageCfgs = sizeof(kStageCfgs) / sizeof(kStageCfgs[0]);
 constexpr int kPinned = 16;   // autotune mode kPinned + c: configuration c pinned
 
@@ -1159,10 +1178,25 @@ static int ensure_colouring(Prepared& P) {
     return SOD_OK;
 }
 
+// Workgroups of the persistent stage kernel: three one-wavefront workgroups
+// per SIMD, the residency its register budget allows.
+static int persist_grid(int nelem) {
+    static int cus = 0;
+    if (cus == 0) {
+        hipDeviceProp_t prop;
+        cus = hipGetDeviceProperties(&prop, rt().device) == hipSuccess ? prop.multiProcessorCount : 256;
+    }
+    return std::max(1, std::min(nelem, cus * 4 * 3));
+}
+
 template <bool HS, int NW, int AS, bool TH>
 static void launch_rhs_t(hipStream_t s, int nelem, const Prepared& P, const double* const in[6], long np,
                          const StageParams& sp) {
-    if (AS == ASM_COLOR) {
+    if constexpr (NW == 0) {
+        hipLaunchKernelGGL((k_stage_rhs<HS, 1, ASM_PULL, TH, true>), dim3(persist_grid(nelem)), dim3(64), 0, s,
+                           nelem, (const int*)nullptr, P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2],
+                           in[3], in[4], in[5], P.relem, np, sp);
+    } else if constexpr (AS == ASM_COLOR) {
         const int nc = (int)P.color_off.size() - 1;
         for (int k = 0; k < nc; ++k) {
             const int b = P.color_off[k], cnt = P.color_off[k + 1] - b;
@@ -1171,11 +1205,11 @@ static void launch_rhs_t(hipStream_t s, int nelem, const Prepared& P, const doub
                                    P.color_elems + b, P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2],
                                    in[3], in[4], in[5], P.rn, np, sp);
         }
-        return;
+    } else {
+        hipLaunchKernelGGL((k_stage_rhs<HS, NW, AS, TH>), dim3(grid_for(nelem, NW)), dim3(64 * NW), 0, s, nelem,
+                           (const int*)nullptr, P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2], in[3],
+                           in[4], in[5], AS == ASM_PULL ? P.relem : P.rn, np, sp);
     }
-    hipLaunchKernelGGL((k_stage_rhs<HS, NW, AS, TH>), dim3(grid_for(nelem, NW)), dim3(64 * NW), 0, s, nelem,
-                       (const int*)nullptr, P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2], in[3], in[4],
-                       in[5], AS == ASM_PULL ? P.relem : P.rn, np, sp);
 }
 
 // stage kernel of configuration `cfg` on the state in[] = rho, q0, q1, q2, E
@@ -1189,6 +1223,7 @@ static void launch_rhs_hs(int cfg, hipStream_t s, int nelem, const Prepared& P, 
         case 2: launch_rhs_t<HS, 4, ASM_PULL, TH>(s, nelem, P, in, np, sp); break;
         case 3: launch_rhs_t<HS, 4, ASM_ATOMIC, TH>(s, nelem, P, in, np, sp); break;
         case 4: launch_rhs_t<HS, 1, ASM_COLOR, TH>(s, nelem, P, in, np, sp); break;
+        case 5: launch_rhs_t<HS, 0, ASM_PULL, TH>(s, nelem, P, in, np, sp); break;
         default: launch_rhs_t<HS, 1, ASM_PULL, TH>(s, nelem, P, in, np, sp); break;
     }
 }
@@ -1334,11 +1369,12 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
     }
 }
 
-static_assert(kNumStageCfgs == 5 && kStageCfgs[0].nw == 1 && kStageCfgs[0].assembly == ASM_PULL &&
+static_assert(kNumStageCfgs == 6 && kStageCfgs[0].nw == 1 && kStageCfgs[0].assembly == ASM_PULL &&
                   kStageCfgs[1].nw == 2 && kStageCfgs[1].assembly == ASM_PULL && kStageCfgs[2].nw == 4 &&
                   kStageCfgs[2].assembly == ASM_PULL && kStageCfgs[3].nw == 4 &&
                   kStageCfgs[3].assembly == ASM_ATOMIC && kStageCfgs[4].nw == 1 &&
-                  kStageCfgs[4].assembly == ASM_COLOR,
+                  kStageCfgs[4].assembly == ASM_COLOR && kStageCfgs[5].nw == 0 &&
+                  kStageCfgs[5].assembly == ASM_PULL,
               "launch_rhs_hs dispatches on these entries");
 
 // Node residual arrays of the scatter configurations (atomic and

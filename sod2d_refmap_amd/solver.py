@@ -24,7 +24,14 @@ class PeriodicSolver:
         init(device)
         self.mesh, self.gas = mesh, gas
         self.cfl_conv, self.cfl_diff = cfl_conv, cfl_diff
-        p = params or numerical_params(Rgas_inf=gas.Rgas)
+        if params is None:
+            # the library's current numerical parameters (flags a caller set
+            # stay in force), with the case's gas constant
+            p = numerical_params()
+            _lib.call("sod_get_numerical_params", p)
+            p.Rgas_inf = gas.Rgas
+        else:
+            p = params
         set_numerical_params(p)
         self.params = p
         t = hex64_tables()

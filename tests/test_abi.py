@@ -86,10 +86,11 @@ def test_stage_configuration_arguments_are_checked():
     for bad in (-1, 3, 16):
         with pytest.raises(_lib.Sod2dError, match="mode"):
             _lib.set_autotune(bad)
-    for nw, asm in ((8, "pull"), (3, "pull"), (1, "atomic"), (2, "atomic"), (4, "colour"), (1, 3)):
+    for nw, asm in ((8, "pull"), (3, "pull"), (1, "atomic"), (2, "atomic"), (4, "colour"), (1, 3), (0, "colour")):
         with pytest.raises(_lib.Sod2dError, match="no configuration"):
             _lib.set_stage_config(nw, asm)
-    for nw, asm in ((1, "pull"), (2, "pull"), (4, "pull"), (4, "atomic"), (1, "colour"), (1, False), (4, True)):
+    for nw, asm in ((1, "pull"), (2, "pull"), (4, "pull"), (4, "atomic"), (1, "colour"), (0, "pull"), (1, False),
+                    (4, True)):
         _lib.set_stage_config(nw, asm)
     for mode in (1, 2, 0):
         _lib.set_autotune(mode)

@@ -18,7 +18,7 @@ from sod2d_refmap_amd.device import numerical_params, set_numerical_params
 pytestmark = pytest.mark.gpu
 
 T = oracle.hex64_tables()
-CFGS = [(4, "pull"), (2, "pull"), (1, "pull"), (4, "atomic"), (1, "colour")]
+CFGS = [(4, "pull"), (2, "pull"), (1, "pull"), (4, "atomic"), (1, "colour"), (0, "pull")]
 KEYS = ("rho", "q", "E", "u", "pr", "Tem", "eta", "csound", "machno", "mu_fluid", "mu_e", "mue_l")
 
 
@@ -107,19 +107,19 @@ def test_autotuned_step_matches_oracle(gpu, default_after, mode):
 def test_time_loop_across_configurations(gpu, default_after):
     """The loop re-captures its step when the configuration changes between
     runs: 2 steps pinned to 1 element per workgroup, 2 with atomic scatter,
-    2 colour-ordered, 2 tuned with every candidate, against 8 default eager
-    steps."""
+    2 colour-ordered, 2 with persistent wavefronts, 2 tuned with every
+    candidate, against 10 default eager steps."""
     from sod2d_refmap_amd.solver import PeriodicSolver
     m, gas, s = _case(perm_seed=13)
     A = PeriodicSolver(m, gas, s)
     B = PeriodicSolver(m, gas, s)
     A.adapt_dt()
     B.adapt_dt()
-    for _ in range(8):
+    for _ in range(10):
         A.rk_4_main(A.dt)
         A.time += A.dt
         A.adapt_dt()
-    for setting in ((1, "pull"), (4, "atomic"), (1, "colour"), "tuned"):
+    for setting in ((1, "pull"), (4, "atomic"), (1, "colour"), (0, "pull"), "tuned"):
         print(f"[time loop] configuration {setting}", flush=True)
         if setting == "tuned":
             _lib.set_autotune(2)
@@ -211,19 +211,27 @@ def test_colouring_is_valid(gpu, default_after, kw):
     gas = cases.GasParams()
     s = cases.tgv_initial_state(m, gas, perturb_seed=2, perturb_amp=2e-2)
     set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    import torch
     _lib.set_stage_config(1, "colour")
+    # one solver (its lumped mass is assembled with atomics, as the
+    # reference does, so a second solver could differ in the last bit)
+    sol = PeriodicSolver(m, gas, s)
+    dt = sol.adapt_dt()
+    tensors = {k: v for k, v in vars(sol).items() if isinstance(v, torch.Tensor)}
+    tensors.update({"lv." + k: v for k, v in sol.lv.items()})
+    saved = {k: v.clone() for k, v in tensors.items()}
     outs = []
     for rep in range(2):
-        sol = PeriodicSolver(m, gas, s)
-        dt = sol.adapt_dt()
+        for k, v in tensors.items():
+            v.copy_(saved[k])
+        torch.cuda.synchronize()
         for _ in range(3):
             sol.rk_4_main(dt)
         outs.append(sol.state())
-        if rep == 0:
-            col = np.zeros(m.nelem, dtype=np.int32)
-            nc = ctypes.c_int(0)
-            _lib.call("sod_element_colouring", None, ctypes.byref(nc), col.ctypes.data, m.nelem)
-        sol.close()
+    col = np.zeros(m.nelem, dtype=np.int32)
+    nc = ctypes.c_int(0)
+    _lib.call("sod_element_colouring", None, ctypes.byref(nc), col.ctypes.data, m.nelem)
+    sol.close()
     conn = np.asarray(m.connec) - 1
     for k in range(nc.value):
         nodes = conn[col == k].ravel()
