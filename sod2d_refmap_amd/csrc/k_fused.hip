@@ -306,6 +306,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
 // Sum of the element-node residuals that belong to node n: the positions
 // p = e*64 + lane of its occurrences in the lane-layout connectivity,
 // ascending (a fixed order, so the assembled residual is reproducible).
+// With inv_off indexed by the working-node position instead of the node id
+// (woff), the same sum needs no lpoin_w load before its offsets.
 __device__ __forceinline__ void node_residuals(int n, const int* __restrict__ inv_off, const int* __restrict__ inv_pos,
                                                const double* __restrict__ relem, double acc[NRES]) {
 #pragma unroll
@@ -358,6 +360,9 @@ struct StepState {
     // lane) positions they are summed over
     const double* relem;
     const int *inv_off, *inv_pos;
+    // the same positions by working-node position it (lpoin_w order):
+    // wpos[woff[it] .. woff[it+1])
+    const int *woff, *wpos;
     // node residual arrays of the atomic-scatter stage kernel (6 x npoin),
     // zeroed again by the update that reads them
     double* Rn;
@@ -410,7 +415,7 @@ __global__ __launch_bounds__(BLK) void k_stage_update(int npoin, int npoin_w, co
                 st.Rn[f * np + n] = 0.0;
             }
         } else {
-            node_residuals(n, st.inv_off, st.inv_pos, st.relem, Rf);
+            node_residuals(it, st.woff, st.wpos, st.relem, Rf);
         }
 #pragma unroll
         for (int f = 0; f < NRES; ++f) Rf[f] = Rf[f] / ml;   // mod_solver.f90:32-36
@@ -780,6 +785,8 @@ struct Prepared {
     double* relem = nullptr;  // nelem*64*6 element-node residuals of the stage kernel
     int* inv_off = nullptr;   // npoin+1: node n's positions are inv_pos[inv_off[n] .. inv_off[n+1])
     int* inv_pos = nullptr;   // nelem*64 positions e*64 + lane, ascending per node
+    int* woff = nullptr;      // npoin_w+1: the offsets of the working node at lpoin_w position it
+    int* wpos = nullptr;      // its positions: inv_pos itself when lpoin_w increases, else a copy
     int* flag = nullptr;
     double* dscal = nullptr;  // device scalars: [0] dt of the step
     double* fpbuf = nullptr;
@@ -806,7 +813,9 @@ struct Prepared {
         (void)hipFree(work);
         (void)hipFree(relem);
         (void)hipFree(inv_off);
+        if (wpos != inv_pos) (void)hipFree(wpos);
         (void)hipFree(inv_pos);
+        (void)hipFree(woff);
         (void)hipFree(flag);
         (void)hipFree(dscal);
         (void)hipFree(fpbuf);
@@ -949,6 +958,11 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
         return fail(SOD_ERR_ARG, "connec holds an id outside 1..npoin");
     }
     if (npoin_w > 0) SOD_HIP(hipMemcpy(hw.data(), lpoin_w, sizeof(int) * npoin_w, hipMemcpyDeviceToHost));
+    for (int t = 0; t < npoin_w; ++t)
+        if (hw[t] < 1 || hw[t] > npoin) {
+            P.release();
+            return fail(SOD_ERR_ARG, "lpoin_w holds an id outside 1..npoin");
+        }
     {
         // node -> (element, lane) positions of the lane-layout connectivity,
         // by a counting sort that keeps the positions ascending
@@ -964,15 +978,33 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
         SOD_HIP(hipMalloc(&P.relem, sizeof(double) * (nl * NRES + 2)));
         SOD_HIP(hipMemcpy(P.inv_off, off.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice));
         if (nl > 0) SOD_HIP(hipMemcpy(P.inv_pos, pos.data(), sizeof(int) * nl, hipMemcpyHostToDevice));
+        // the same lists in lpoin_w order
+        std::vector<int> woff((size_t)npoin_w + 1, 0);
+        bool increasing = true;
+        for (int t = 0; t < npoin_w; ++t) {
+            const int n = hw[t] - 1;
+            woff[t + 1] = woff[t] + (off[n + 1] - off[n]);
+            if (t > 0 && hw[t] <= hw[t - 1]) increasing = false;
+        }
+        SOD_HIP(hipMalloc(&P.woff, sizeof(int) * ((size_t)npoin_w + 1)));
+        if (increasing && (size_t)woff[npoin_w] == nl) {
+            // the working nodes' lists follow each other in inv_pos (nodes
+            // outside lpoin_w hold no positions: connec never names them)
+            for (int t = 0; t <= npoin_w; ++t) woff[t] += npoin_w > 0 ? off[hw[0] - 1] : 0;
+            P.wpos = P.inv_pos;
+        } else {
+            std::vector<int> wp((size_t)woff[npoin_w] + 1);
+            for (int t = 0; t < npoin_w; ++t) {
+                const int n = hw[t] - 1;
+                std::copy(pos.begin() + off[n], pos.begin() + off[n + 1], wp.begin() + woff[t]);
+            }
+            SOD_HIP(hipMalloc(&P.wpos, sizeof(int) * wp.size()));
+            SOD_HIP(hipMemcpy(P.wpos, wp.data(), sizeof(int) * wp.size(), hipMemcpyHostToDevice));
+        }
+        SOD_HIP(hipMemcpy(P.woff, woff.data(), sizeof(int) * woff.size(), hipMemcpyHostToDevice));
     }
     std::vector<char> isw(npoin, 0);
-    for (int t = 0; t < npoin_w; ++t) {
-        if (hw[t] < 1 || hw[t] > npoin) {
-            P.release();
-            return fail(SOD_ERR_ARG, "lpoin_w holds an id outside 1..npoin");
-        }
-        isw[hw[t] - 1] = 1;
-    }
+    for (int t = 0; t < npoin_w; ++t) isw[hw[t] - 1] = 1;
     std::vector<int> sl;
     for (int t = 0; t < npoin; ++t)
         if (!isw[t]) sl.push_back(t);
@@ -1297,6 +1329,8 @@ static StepState step_state(Prepared& P, const FusedArgs& a) {
     st.relem = P.relem;
     st.inv_off = P.inv_off;
     st.inv_pos = P.inv_pos;
+    st.woff = P.woff;
+    st.wpos = P.wpos;
     st.Rn = P.rn;
     st.hmin = nullptr;
     st.dt_bits = nullptr;
