@@ -46,7 +46,9 @@ enum {
     L_UA = 47                                                     // 3: U^c of the lane
 };
 
-template <bool HAS_SGS>
+// SUTH: the Sutherland gate (P.suth) as a compile-time choice; as a run-time
+// branch it made the compiler spill ~290 VGPRs to scratch (13x slower).
+template <bool HAS_SGS, bool SUTH>
 __global__ __launch_bounds__(64 * AWPB) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_stage_rhs_vjp(
     int nelem, int npoin, const int* __restrict__ conn, const double* __restrict__ He,
     const double* __restrict__ gpv, const double* __restrict__ mue, const double* __restrict__ musgs,
@@ -85,7 +87,7 @@ __global__ __launch_bounds__(64 * AWPB) __attribute__((amdgpu_waves_per_eu(2, 8)
     const double sT = sqrt(T);
     // Sutherland's law, or the caller's mu_fluid (a constant of the state)
     // with the gate off
-    const double muf = P.suth ? mf * 0.000001458 * (T * sT) / (T + 110.4) : mf;
+    const double muf = SUTH ? mf * 0.000001458 * (T * sT) / (T + 110.4) : mf;
     double hm[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(64 * AWPB) __attribute__((amdgpu_waves_per_eu(2, 8)
     {
         double LT = Lp_T, Lr = Lp_r, Lp = Lp_p, Lri = L_ri, Lk = Lp_k;
         const double den = T + 110.4;
-        if (P.suth) LT += L_muf * mf * 0.000001458 * (1.5 * sT * den - T * sT) / (den * den);
+        if (SUTH) LT += L_muf * mf * 0.000001458 * (1.5 * sT * den - T * sT) / (den * den);
         Lr += Lp_eta * P.ig1 * (lphi - P.gamma);
         Lp += Lp_eta * r * P.ig1 / p;
         Lp += LT * ri * P.iRgas + Lp_h * P.gog1 * ri;
@@ -793,14 +795,10 @@ extern "C" int sod_full_rhs_vjp(int nelem, int npoin, int npoin_w, const int* co
     const StageParams sp = make_stage_params(gamma_gas, Rgas, Cp, Prt, prm);
     SOD_PROF("full_rhs_vjp");   // the element kernel, the mu_e transpose and the node sums
     if (nelem > 0) {
-        if (lm.has_sgs)
-            hipLaunchKernelGGL(k_stage_rhs_vjp<true>, dim3(grid_for(nelem, AWPB)), dim3(64 * AWPB), 0, s, nelem,
-                               npoin, lm.conn, lm.He, lm.gpv, lm.mue, lm.musgs, dr, dq, dq + np, dq + 2 * np, dE, dmf,
-                               lam, lm.relem, gml, sp);
-        else
-            hipLaunchKernelGGL(k_stage_rhs_vjp<false>, dim3(grid_for(nelem, AWPB)), dim3(64 * AWPB), 0, s, nelem,
-                               npoin, lm.conn, lm.He, lm.gpv, lm.mue, lm.musgs, dr, dq, dq + np, dq + 2 * np, dE, dmf,
-                               lam, lm.relem, gml, sp);
+        auto* kern = lm.has_sgs ? (sp.suth ? k_stage_rhs_vjp<true, true> : k_stage_rhs_vjp<true, false>)
+                                : (sp.suth ? k_stage_rhs_vjp<false, true> : k_stage_rhs_vjp<false, false>);
+        hipLaunchKernelGGL(kern, dim3(grid_for(nelem, AWPB)), dim3(64 * AWPB), 0, s, nelem, npoin, lm.conn, lm.He,
+                           lm.gpv, lm.mue, lm.musgs, dr, dq, dq + np, dq + 2 * np, dE, dmf, lam, lm.relem, gml, sp);
         hipLaunchKernelGGL(k_lane_to_ref_tiled, dim3(grid_for(nelem, 16)), dim3(256), 0, s, nelem, gml, dgmu);
     }
     if (npoin > 0)

@@ -152,6 +152,11 @@ def test_smart_visc_vjp_matches_autograd(gpu):
         assert err < 1e-11, (name, err)
 
 
+def fd4(f, h):
+    """Fourth-order central difference of the scalar map a -> f(a) at 0."""
+    return (8.0 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12.0 * h)
+
+
 def _ev_context(m, g, gas, s):
     from sod2d_refmap_amd.autograd import EVContext, RHSContext
     rc = RHSContext.from_mesh(m, g["He"], g["gpvol"], g["Ml"], s["mu_factor"], gas)
@@ -193,13 +198,16 @@ def test_rk4_chain_matches_rk_4_main(gpu):
 
 def test_smart_visc_vjp_alone_against_frozen_branches(gpu):
     """The entropy-viscosity adjoint (sod_smart_visc_spectral_vjp) alone,
-    against central differences of smart_visc_spectral along the branches
-    recorded at the base state (frozen_visc.py), in all six inputs at once,
-    with the residual mixing signs and saturating a seventh of the nodes:
-    relative 1e-8 (a kink would show as an O(1) difference, a wrong
+    against fourth-order central differences of smart_visc_spectral along the
+    branches recorded at the base state (frozen_visc.py), in all six inputs
+    at once, with the residual mixing signs and saturating a seventh of the
+    nodes: relative 1e-8 (a kink would show as an O(1) difference, a wrong
     derivative of a smooth branch as a difference that does not shrink with
-    the step).  The recorded-branch map equals the library's forward at the
-    base state."""
+    the step).  The map is strongly curved along these directions (the
+    normalisation max |eta - mean| is ~0.04 here): a second-order quotient at
+    step 1e-7 carries ~1e-8 of truncation error, the fourth-order one ~1e-11
+    (both measured with the torch transcription and autograd on the CPU).
+    The recorded-branch map equals the library's forward at the base state."""
     from frozen_visc import FrozenSmartVisc
     from sod2d_refmap_amd.autograd import SmartVisc
     from sod2d_refmap_amd.device import numerical_params, set_numerical_params
@@ -229,9 +237,8 @@ def test_smart_visc_vjp_alone_against_frozen_branches(gpu):
             v = [sc * torch.tensor(rng.standard_normal(tuple(t.shape)), dtype=torch.float64, device="cuda")
                  for sc, t in zip(scale, base)]
             with torch.no_grad():
-                fp = float((fz(*[b + h * d for b, d in zip(base, v)], branches=br)[0] * lam).sum())
-                fm = float((fz(*[b - h * d for b, d in zip(base, v)], branches=br)[0] * lam).sum())
-            fd = (fp - fm) / (2 * h)
+                f = lambda a: float((fz(*[b + a * d for b, d in zip(base, v)], branches=br)[0] * lam).sum())
+                fd = fd4(f, h)
             an = float(sum((gg * d).sum() for gg, d in zip(grads, v)))
             print(f"  smart_visc vjp (normalise {normalise}) direction {it}: fd={fd:.12e} adjoint={an:.12e}",
                   flush=True)
@@ -245,7 +252,12 @@ def test_rk4_adjoint_checkpointed_fd(gpu):
     chain (same dt sequence) along seeded random unit directions, with the
     entropy normalisation on and off.  The +-h chains follow the entropy
     viscosity's branches recorded on the unperturbed chain (frozen_visc.py),
-    the map the adjoint differentiates: 1e-6 relative."""
+    the map the adjoint differentiates: 1e-6 relative, with fourth-order
+    central differences at step 1e-4 along the recorded branches.  At step
+    1e-6 the objective's FP64 rounding (J ~ 3) puts up to 9e-7 of noise into
+    the quotient where the directional derivative is small (7e-4); at 1e-4
+    the quotient agrees with autograd through the torch transcription of the
+    same chain to 1e-8 (measured on the CPU)."""
     from frozen_visc import FrozenSmartVisc, trajectory
     from sod2d_refmap_amd.autograd import kinetic_energy, run_checkpointed
     for normalise in (0, 1):
@@ -285,11 +297,61 @@ def _rk4_adjoint_fd(normalise, FrozenSmartVisc, trajectory, kinetic_energy, run_
     Jv = float(J.detach())
     assert abs(forward(u0) - Jv) <= 1e-13 * abs(Jv)
     rng = np.random.default_rng(32)
-    h = 1e-6
+    h = 1e-4
     for it in range(4):
         v = torch.tensor(rng.standard_normal((m.npoin, 3)), dtype=torch.float64, device="cuda")
         v = v / torch.linalg.norm(v)
-        fd = (forward(u0 + h * v) - forward(u0 - h * v)) / (2 * h)
+        fd = fd4(lambda a: forward(u0 + a * v), h)
         an = float((gu * v).sum())
         print(f"  rk4 adjoint (normalise {normalise}) direction {it}: fd={fd:.12e} adjoint={an:.12e}", flush=True)
         assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+def test_vjp_costs_at_most_three_forward_rhs(gpu):
+    """sod_full_rhs_vjp against sod_full_rhs on a 32^3 Taylor-Green box
+    (32,768 elements): mean time per call from HIP events on the library
+    stream over 5 calls each after a warm-up.  The adjoint recomputes the
+    forward element work and runs three transposed line passes; it must stay
+    within 3x a forward evaluation (a register-spilling build measured 13x)."""
+    import ctypes
+    from sod2d_refmap_amd.device import numerical_params, set_numerical_params
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    m = pmesh.periodic_box(32)
+    gas = cases.GasParams()
+    s = cases.tgv_initial_state(m, gas, perturb_seed=3, perturb_amp=1e-2)
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    sol = PeriodicSolver(m, gas, s)
+    n, ne = m.npoin, m.nelem
+    z = lambda k: torch.zeros(k, dtype=torch.float64, device="cuda")
+    common = [sol.connec.data_ptr(), sol.lpoin_w.data_ptr(), sol.He.data_ptr(), sol.gpvol.data_ptr(),
+              sol.Ml.data_ptr(), sol.level("rho").data_ptr(), sol.level("q").data_ptr(), sol.level("E").data_ptr(),
+              sol.mu_factor.data_ptr(), sol.mu_e.data_ptr(), sol.mu_sgs.data_ptr(), gas.Rgas, gas.gamma_gas, gas.Cp,
+              gas.Prt]
+    outs = [z(n), z(3 * n), z(n), z(n)]
+    lam = [torch.randn(k * n, dtype=torch.float64, device="cuda") for k in (1, 3, 1, 1)]
+    gout = [z(n), z(3 * n), z(n), z(64 * ne)]
+    fwd = lambda: _lib.call("sod_full_rhs", ne, n, m.npoin_w, *common, *[o.data_ptr() for o in outs])
+    adj = lambda: _lib.call("sod_full_rhs_vjp", ne, n, m.npoin_w, *common, *[x.data_ptr() for x in lam],
+                            *[o.data_ptr() for o in gout])
+    fwd()
+    adj()
+    torch.cuda.synchronize()
+    _lib.call("sod_profile_enable", 1)
+    _lib.call("sod_profile_reset")
+    try:
+        for _ in range(5):
+            fwd()
+            adj()
+        torch.cuda.synchronize()
+        t = {}
+        for name in ("full_rhs", "full_rhs_vjp"):
+            tot, cnt = ctypes.c_double(0.0), ctypes.c_int(0)
+            _lib.call("sod_profile_get", name.encode(), ctypes.byref(tot), ctypes.byref(cnt))
+            t[name] = tot.value / max(cnt.value, 1)
+    finally:
+        _lib.call("sod_profile_enable", 0)
+    ratio = t["full_rhs_vjp"] / t["full_rhs"]
+    print(f"[vjp cost] 32^3: forward {t['full_rhs']:.4f} ms, vjp {t['full_rhs_vjp']:.4f} ms, ratio {ratio:.2f}",
+          flush=True)
+    sol.close()
+    assert ratio <= 3.0, ratio
