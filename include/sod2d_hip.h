@@ -197,6 +197,25 @@ int sod_smart_visc_spectral(int nelem, int npoin, int npoin_w, const int *connec
                             const double *Ml, double *mu_e, const int *invAtoIJK,
                             const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
                             double *mue_l);
+/* No reference counterpart.  sod_smart_visc_spectral for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same arguments, kernels and results;
+ * its reduction scratch (the entropy normalisation's block partials, their
+ * sum and the normalisation) lives in a built-in device array of 16 KiB per
+ * GPU, held until the process ends, so the call allocates nothing.  The
+ * array is shared by every call on one GPU: calls that overlap on one GPU
+ * race.  Requires sod_init; a workload needing more scratch than the array
+ * holds is refused with SOD_ERR_ARG (the entropy viscosity needs a fixed
+ * 8.3 KiB at any mesh size). */
+int sod_smart_visc_spectral_noalloc(int nelem, int npoin, int npoin_w, const int *connec,
+                                    const int *lpoin_w, const double *Reta, const double *Ngp,
+                                    const double *coord, const double *dNgp, const double *gpvol,
+                                    const double *wgp, double gamma_gas, const double *rho,
+                                    const double *u, const double *csound, const double *Tem,
+                                    const double *eta, const double *helem, const double *helem_k,
+                                    const double *Ml, double *mu_e, const int *invAtoIJK,
+                                    const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                                    double *mue_l);
 /* adapt_dt_cfl, mod_time_ops.f90:11-61.  `dt` is a HOST scalar (inout).  The
  * optional arguments cfl_diff/mu_fluid/mu_sgs/rho are all given or all NULL
  * (cfl_diff is passed by address for that reason). */

@@ -65,6 +65,7 @@ _SIGS = {
     "sod_lumped_solver_scal": [c_int, c_int, P, P, P],
     "sod_lumped_solver_vect": [c_int, c_int, P, P, P],
     "sod_smart_visc_spectral": [c_int, c_int, c_int] + [P] * 8 + [c_double] + [P] * 14,
+    "sod_smart_visc_spectral_noalloc": [c_int, c_int, c_int] + [P] * 8 + [c_double] + [P] * 14,
     "sod_adapt_dt_cfl": [c_int, c_int, P, P, P, P, c_double, P, P, P, P, P],
     "sod_volAvg_EK": [c_int, c_int, P, P, P, c_double, P, P, P],
     "sod_set_rk4_path": [c_int],

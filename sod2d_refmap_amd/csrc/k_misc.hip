@@ -604,6 +604,8 @@ void launch_lumped(int npoin, int npoin_w, int ncomp, const int* lpoin_w, const 
 }
 
 // Entropy normalisation + element loop. `work` holds >= RED_BLOCKS + 2 doubles.
+size_t smart_visc_scratch_doubles() { return RED_BLOCKS + 8; }
+
 void launch_smart_visc(int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
                        const double* Reta, const double* gpvol, double gamma_gas, const double* rho,
                        const double* u, const double* csound, const double* Tem, const double* eta,

@@ -18,6 +18,7 @@ void fused_release();     // k_fused.hip
 void dt_cache_release();  // k_misc.hip
 int fused_invalidate(const void* lo, size_t bytes);      // k_fused.hip
 int dt_cache_invalidate(const void* lo, size_t bytes);   // k_misc.hip
+int noalloc_setup();      // k_noalloc.hip
 
 namespace {
 thread_local char g_err[2048] = "";
@@ -124,9 +125,33 @@ static int copy_to_host(void* dst, const void* src, size_t bytes) {
 // The kernels use the library's own p=3 tables; the caller's tables must be
 // the reference's tables (a2ijk = gmsh2ijk, FP32-rounded GLL nodes).  Integer
 // tables must match exactly, the derivative table to 1e-12 relative.
+// One caller table against the library's: kind 0 the derivative table
+// dlxigp_ip (1e-12 relative), 1-4 invAtoIJK, gmshAtoI/J/K (exact).
+static int check_table(const void* p, int kind) {
+    const HostTables& t = host_tables();
+    const void* d = nullptr;
+    SOD_TRY(resolve(p, "table", &d));
+    if (kind == 0) {
+        double h[768];
+        SOD_TRY(copy_to_host(h, d, sizeof(h)));
+        for (int i = 0; i < 768; ++i) {
+            double ref = t.dlxigp_ip[i];
+            if (std::fabs(h[i] - ref) > 1e-12 * (1.0 + std::fabs(ref)))
+                return fail(SOD_ERR_ARG, "dlxigp_ip differs from the p=3 GLL table at %d (%.17g vs %.17g)", i, h[i],
+                            ref);
+        }
+    } else {
+        int h[64];
+        SOD_TRY(copy_to_host(h, d, sizeof(h)));
+        const int* ref = kind == 1 ? t.invAtoIJK : kind == 2 ? t.gmshAtoI : kind == 3 ? t.gmshAtoJ : t.gmshAtoK;
+        if (std::memcmp(h, ref, sizeof(h)) != 0)
+            return fail(SOD_ERR_ARG, "node-order table %d differs from a2ijk = gmsh2ijk for p=3", kind);
+    }
+    return SOD_OK;
+}
+
 int check_tables(const double* dlxigp_ip, const int* invAtoIJK, const int* gmshAtoI,
                  const int* gmshAtoJ, const int* gmshAtoK) {
-    const HostTables& t = host_tables();
     struct Item {
         const void* p;
         int kind;
@@ -137,29 +162,18 @@ int check_tables(const double* dlxigp_ip, const int* invAtoIJK, const int* gmshA
             return fail(SOD_ERR_ARG, "a p=3 table argument is NULL");
         }
         if (g_tables_ok.count({it.p, it.kind})) continue;
-        const void* d = nullptr;
-        SOD_TRY(resolve(it.p, "table", &d));
-        if (it.kind == 0) {
-            double h[768];
-            SOD_TRY(copy_to_host(h, d, sizeof(h)));
-            for (int i = 0; i < 768; ++i) {
-                double ref = t.dlxigp_ip[i];
-                if (std::fabs(h[i] - ref) > 1e-12 * (1.0 + std::fabs(ref)))
-                    return fail(SOD_ERR_ARG,
-                                "dlxigp_ip differs from the p=3 GLL table at %d (%.17g vs %.17g)",
-                                i, h[i], ref);
-            }
-        } else {
-            int h[64];
-            SOD_TRY(copy_to_host(h, d, sizeof(h)));
-            const int* ref = it.kind == 1 ? t.invAtoIJK
-                             : it.kind == 2 ? t.gmshAtoI
-                             : it.kind == 3 ? t.gmshAtoJ : t.gmshAtoK;
-            if (std::memcmp(h, ref, sizeof(h)) != 0)
-                return fail(SOD_ERR_ARG,
-                            "node-order table %d differs from a2ijk = gmsh2ijk for p=3", it.kind);
-        }
+        SOD_TRY(check_table(it.p, it.kind));
         g_tables_ok.insert({it.p, it.kind});
+    }
+    return SOD_OK;
+}
+
+int check_tables_noalloc(const int* invAtoIJK, const int* gmshAtoI, const int* gmshAtoJ, const int* gmshAtoK) {
+    const void* items[4] = {invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK};
+    for (int k = 0; k < 4; ++k) {
+        if (items[k] == nullptr) return fail(SOD_ERR_ARG, "a p=3 table argument is NULL");
+        if (g_tables_ok.count({items[k], k + 1})) continue;
+        SOD_TRY(check_table(items[k], k + 1));
     }
     return SOD_OK;
 }
@@ -197,6 +211,9 @@ int sod_init(int porder) {
     if (ndev < 1) return fail(SOD_ERR_HIP, "no HIP device visible");
     SOD_HIP(hipGetDevice(&r.device));
     SOD_TRY(upload_tables());
+    // the allocation-free entry points' scratch on this device (they report
+    // a failure of this lookup; nothing else depends on it)
+    (void)noalloc_setup();
     sod_default_numerical_params(&r.params);
     r.stream = nullptr;
     r.initialised = true;

@@ -124,6 +124,9 @@ inline int resolve_w(T* p, const char* what, T** out) {
 // Checks caller-supplied p=3 tables against the library's (once per pointer).
 int check_tables(const double* dlxigp_ip, const int* invAtoIJK, const int* gmshAtoI,
                  const int* gmshAtoJ, const int* gmshAtoK);
+// The node-order tables checked without remembering the result (no host
+// allocation; pointers already checked by check_tables are not read again).
+int check_tables_noalloc(const int* invAtoIJK, const int* gmshAtoI, const int* gmshAtoJ, const int* gmshAtoK);
 
 // Device scratch owned by the runtime, grown on demand.
 int scratch(size_t bytes, void** out, int slot);
