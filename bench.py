@@ -111,6 +111,12 @@ def measured_traffic(kernel: str, n: int):
     return None
 
 
+def kernel_symbol(kernel: str) -> str:
+    """The HIP kernel behind a profiling range name."""
+    return {"stage_rhs": "k_stage_rhs", "stage_update": "k_stage_update",
+            "smart_visc_fused": "k_smart_visc_lane"}.get(kernel, kernel)
+
+
 def measured_flops_per_wave(kernel: str, n: int):
     """FP64 operations per wavefront (one element) of `kernel`: 2 per FMA,
     1 per add or multiply, from the SQ instruction counters of the committed
@@ -119,9 +125,13 @@ def measured_flops_per_wave(kernel: str, n: int):
     if not os.path.exists(path):
         return None
     per_wave = {}
+    section = None
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0].startswith("SQ_INSTS_VALU_"):
+        if line.startswith("kernel "):
+            section = line.split(None, 1)[1].strip()
+        elif section is not None and section.startswith(kernel_symbol(kernel)) and len(f) == 3 \
+                and f[0].startswith("SQ_INSTS_VALU_"):
             per_wave[f[0]] = float(f[2])
     try:
         return 64 * (2 * per_wave["SQ_INSTS_VALU_FMA_F64"] + per_wave["SQ_INSTS_VALU_ADD_F64"] +
