@@ -226,8 +226,8 @@ def main():
     ap.add_argument("--batch", type=int, default=48,
                     help="also time the seeded ensemble of this many boxes (0: skip)")
     ap.add_argument("--stage-config", default="",
-                    help="pin the stage kernel configuration as ELEMS_PER_WORKGROUP,ASSEMBLY "
-                         "(assembly pull|atomic|colour; 0,pull: persistent wavefronts)")
+                    help="pin the stage kernel configuration as ELEMS_PER_WORKGROUP,ASSEMBLY[,LDS] "
+                         "(assembly pull|atomic|colour; 0,pull: persistent wavefronts; LDS swizzled|linear)")
     ap.add_argument("--autotune", type=int, default=0, help="stage configuration mode (sod_set_autotune)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="start with one 1 GiB device-to-device copy (calibrates HBM byte counters)")
@@ -246,8 +246,8 @@ def main():
     from sod2d_refmap_amd.device import init as dev_init
     dev_init(0)
     if args.stage_config:
-        nw, asm = args.stage_config.split(",")
-        _lib.set_stage_config(int(nw), asm)
+        nw, asm, *lay = args.stage_config.split(",")
+        _lib.set_stage_config(int(nw), asm, lay[0] if lay else "swizzled")
     elif args.autotune:
         _lib.set_autotune(args.autotune)
     if args.copy_probe:

@@ -399,6 +399,15 @@ int sod_set_autotune(int mode);
  * A pinned colour-ordered configuration on a mesh that cannot be coloured
  * makes the stepping call return SOD_ERR_UNSUPPORTED. */
 int sod_set_stage_config(int elems_per_workgroup, int assembly);
+/* No reference counterpart: sod_set_stage_config with the LDS layout of the
+ * element's nodal values as a third dimension: 0 swizzled slots (the
+ * default), 1 the lane index itself (1 element per workgroup with assembly 0
+ * only).  The autotuner's modes 1 and 2 time both layouts. */
+int sod_set_stage_config_lds(int elems_per_workgroup, int assembly, int lds_layout);
+/* No reference counterpart: the LDS layout (0 swizzled, 1 linear) of the
+ * stage configuration of loop's mesh (NULL: the mesh most recently stepped
+ * through sod_rk_4_main). */
+int sod_stage_lds_layout(const sod_time_loop *loop, int *lds_layout);
 /* The stage configuration of loop's mesh (loop NULL: the mesh most recently
  * stepped through sod_rk_4_main): elements per workgroup, assembly (0-2 as
  * above), the mode it was chosen under (0-2, or 3 when pinned) and the

@@ -83,6 +83,8 @@ _SIGS = {
     "sod_time_loop_run_batched": [c_void_p, c_int, c_double, c_double, c_int, P, P, c_int],
     "sod_set_autotune": [c_int],
     "sod_set_stage_config": [c_int, c_int],
+    "sod_set_stage_config_lds": [c_int, c_int, c_int],
+    "sod_stage_lds_layout": [c_void_p, ctypes.POINTER(c_int)],
     "sod_element_colouring": [c_void_p, ctypes.POINTER(c_int), P, c_int],
     "sod_stage_config": [c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                          ctypes.POINTER(c_double)],
@@ -127,6 +129,8 @@ def call(name: str, *args) -> None:
 
 ASSEMBLY = {"pull": 0, "atomic": 1, "colour": 2}
 _ASSEMBLY_NAMES = {v: k for k, v in ASSEMBLY.items()}
+LDS_LAYOUT = {"swizzled": 0, "linear": 1}
+_LDS_NAMES = {v: k for k, v in LDS_LAYOUT.items()}
 
 
 def set_autotune(mode: int) -> None:
@@ -138,21 +142,25 @@ def set_autotune(mode: int) -> None:
     call("sod_set_autotune", int(mode))
 
 
-def set_stage_config(elems_per_workgroup: int, assembly) -> None:
-    """Pins one stage configuration (sod_set_stage_config).  assembly:
+def set_stage_config(elems_per_workgroup: int, assembly, lds="swizzled") -> None:
+    """Pins one stage configuration (sod_set_stage_config_lds).  assembly:
     "pull" (element-node stores), "atomic" or "colour", or the ABI's 0/1/2
-    (a bool keeps its meaning of atomic scatter or not)."""
+    (a bool keeps its meaning of atomic scatter or not); lds: "swizzled" or
+    "linear" (or 0/1), the LDS layout of the element's nodal values."""
     a = ASSEMBLY[assembly] if isinstance(assembly, str) else int(assembly)
-    call("sod_set_stage_config", int(elems_per_workgroup), a)
+    lay = LDS_LAYOUT[lds] if isinstance(lds, str) else int(lds)
+    call("sod_set_stage_config_lds", int(elems_per_workgroup), a, lay)
 
 
 def stage_config(loop=None) -> dict:
     """The stage configuration of a time loop's mesh, or (loop None) of the
     mesh last stepped through sod_rk_4_main (sod_stage_config)."""
-    nw, at, md, ms = c_int(), c_int(), c_int(), c_double()
+    nw, at, md, ms, lay = c_int(), c_int(), c_int(), c_double(), c_int()
     call("sod_stage_config", loop, ctypes.byref(nw), ctypes.byref(at), ctypes.byref(md), ctypes.byref(ms))
+    call("sod_stage_lds_layout", loop, ctypes.byref(lay))
     return {"elems_per_workgroup": nw.value, "assembly": _ASSEMBLY_NAMES.get(at.value, str(at.value)),
-            "atomic_scatter": at.value == 1, "mode": md.value, "stage_ms": ms.value}
+            "atomic_scatter": at.value == 1, "lds": _LDS_NAMES.get(lay.value, str(lay.value)), "mode": md.value,
+            "stage_ms": ms.value}
 
 
 def exported_symbols() -> list[str]:

@@ -64,7 +64,9 @@ constexpr int BLK = 256;
 // loads the next element's node ids after the current element's own loads,
 // so that an element waits for one round of loads (its gathers, issued with
 // its geometry) instead of two (the node ids, then the gathers).
-template <bool HAS_SGS, int NW, int ASM, bool TOTAL_H, bool PERSIST = false>
+// LAY: the LDS slot of a lattice point, swizzled (LDS_SWIZZLED, the default)
+// or the lane index itself (LDS_LINEAR); an autotuner candidate.
+template <bool HAS_SGS, int NW, int ASM, bool TOTAL_H, bool PERSIST = false, int LAY = LDS_SWIZZLED>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_stage_rhs(
     int nelem, const int* __restrict__ elist, const int* __restrict__ conn, const double* __restrict__ He,
     const double* __restrict__ gpv, const double* __restrict__ mue, const double* __restrict__ musgs,
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
     if (PERSIST && e_next < nelem) n_next = conn[(size_t)e_next * 64 + l];
 
     double(*S)[64] = sh[w];
-    const int me = swz(i, j, k);
+    const int me = lds_slot<LAY>(i, j, k);
     // the convected energy variable: h + k (full_convec_ijk), or H = E + p
     // (full_convec_ijk_H: fe = (E+p)u, gradE = grad(p+E))
     const double hk = TOTAL_H ? en + p : h + kk;
@@ -173,7 +175,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
                 rb = r; u0b = u0; u1b = u1; u2b = u2; pb = p; hkb = hk; etab = eta; Tb = T;
             } else {
                 const int bb = (o + d) & 3;
-                const int sl = c == 0 ? swz(bb, j, k) : (c == 1 ? swz(i, bb, k) : swz(i, j, bb));
+                const int sl = c == 0 ? lds_slot<LAY>(bb, j, k)
+                                      : (c == 1 ? lds_slot<LAY>(i, bb, k) : lds_slot<LAY>(i, j, bb));
                 rb = S[P_RHO][sl]; u0b = S[P_U0][sl]; u1b = S[P_U1][sl]; u2b = S[P_U2][sl];
                 pb = S[P_P][sl]; hkb = S[P_H][sl]; etab = S[P_ETA][sl]; Tb = S[P_T][sl];
             }
@@ -269,7 +272,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
                 for (int f = 0; f < 6; ++f) dD[f] += Dt * G[f][c];
             } else {
                 const int bb = (o + d) & 3;
-                const int sl = c == 0 ? swz(bb, j, k) : (c == 1 ? swz(i, bb, k) : swz(i, j, bb));
+                const int sl = c == 0 ? lds_slot<LAY>(bb, j, k)
+                                      : (c == 1 ? lds_slot<LAY>(i, bb, k) : lds_slot<LAY>(i, j, bb));
 #pragma unroll
                 for (int f = 0; f < 6; ++f) dD[f] += Dt * S[3 * f + c][sl];
             }
@@ -1142,12 +1146,17 @@ struct DtFold {
 // the caller's mesh and keep the fastest.
 // nw == 0: the persistent one-wavefront kernel (k_stage_rhs<..., PERSIST>)
 // with element-node stores.
+// lds: the LDS slot layout of the element's nodal values (LDS_SWIZZLED or
+// LDS_LINEAR).
 struct StageCfg {
     int nw;
     int assembly;   // ASM_PULL, ASM_ATOMIC, ASM_COLOR
+    int lds;
 };
-constexpr StageCfg kStageCfgs[] = {{1, ASM_PULL}, {2, ASM_PULL}, {4, ASM_PULL}, {4, ASM_ATOMIC}, {1, ASM_COLOR},
-                                   {0, ASM_PULL}};
+constexpr StageCfg kStageCfgs[] = {{1, ASM_PULL, LDS_SWIZZLED},  {2, ASM_PULL, LDS_SWIZZLED},
+                                   {4, ASM_PULL, LDS_SWIZZLED},  {4, ASM_ATOMIC, LDS_SWIZZLED},
+                                   {1, ASM_COLOR, LDS_SWIZZLED}, {0, ASM_PULL, LDS_SWIZZLED},
+                                   {1, ASM_PULL, LDS_LINEAR}};
 constexpr int kNumStageCfgs = sizeof(kStageCfgs) / sizeof(kStageCfgs[0]);
 constexpr int kPinned = 16;   // autotune mode kPinned + c: configuration c pinned
 
@@ -1221,7 +1230,7 @@ static int persist_grid(int nelem) {
     return std::max(1, std::min(nelem, cus * 4 * 3));
 }
 
-template <bool HS, int NW, int AS, bool TH>
+template <bool HS, int NW, int AS, bool TH, int LAY = LDS_SWIZZLED>
 static void launch_rhs_t(hipStream_t s, int nelem, const Prepared& P, const double* const in[6], long np,
                          const StageParams& sp) {
     if constexpr (NW == 0) {
@@ -1238,7 +1247,8 @@ static void launch_rhs_t(hipStream_t s, int nelem, const Prepared& P, const doub
                                    in[3], in[4], in[5], P.rn, np, sp);
         }
     } else {
-        hipLaunchKernelGGL((k_stage_rhs<HS, NW, AS, TH>), dim3(grid_for(nelem, NW)), dim3(64 * NW), 0, s, nelem,
+        hipLaunchKernelGGL((k_stage_rhs<HS, NW, AS, TH, false, LAY>), dim3(grid_for(nelem, NW)), dim3(64 * NW), 0, s,
+                           nelem,
                            (const int*)nullptr, P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2], in[3],
                            in[4], in[5], AS == ASM_PULL ? P.relem : P.rn, np, sp);
     }
@@ -1256,6 +1266,7 @@ static void launch_rhs_hs(int cfg, hipStream_t s, int nelem, const Prepared& P, 
         case 3: launch_rhs_t<HS, 4, ASM_ATOMIC, TH>(s, nelem, P, in, np, sp); break;
         case 4: launch_rhs_t<HS, 1, ASM_COLOR, TH>(s, nelem, P, in, np, sp); break;
         case 5: launch_rhs_t<HS, 0, ASM_PULL, TH>(s, nelem, P, in, np, sp); break;
+        case 6: launch_rhs_t<HS, 1, ASM_PULL, TH, LDS_LINEAR>(s, nelem, P, in, np, sp); break;
         default: launch_rhs_t<HS, 1, ASM_PULL, TH>(s, nelem, P, in, np, sp); break;
     }
 }
@@ -1403,12 +1414,13 @@ static void enqueue_fused_step(Prepared& P, const FusedArgs& a, const double* d_
     }
 }
 
-static_assert(kNumStageCfgs == 6 && kStageCfgs[0].nw == 1 && kStageCfgs[0].assembly == ASM_PULL &&
+static_assert(kNumStageCfgs == 7 && kStageCfgs[0].nw == 1 && kStageCfgs[0].assembly == ASM_PULL &&
                   kStageCfgs[1].nw == 2 && kStageCfgs[1].assembly == ASM_PULL && kStageCfgs[2].nw == 4 &&
                   kStageCfgs[2].assembly == ASM_PULL && kStageCfgs[3].nw == 4 &&
                   kStageCfgs[3].assembly == ASM_ATOMIC && kStageCfgs[4].nw == 1 &&
                   kStageCfgs[4].assembly == ASM_COLOR && kStageCfgs[5].nw == 0 &&
-                  kStageCfgs[5].assembly == ASM_PULL,
+                  kStageCfgs[5].assembly == ASM_PULL && kStageCfgs[6].nw == 1 &&
+                  kStageCfgs[6].assembly == ASM_PULL && kStageCfgs[6].lds == LDS_LINEAR,
               "launch_rhs_hs dispatches on these entries");
 
 // Node residual arrays of the scatter configurations (atomic and
@@ -2063,15 +2075,21 @@ extern "C" int sod_set_autotune(int mode) {
     return SOD_OK;
 }
 
-extern "C" int sod_set_stage_config(int elems_per_workgroup, int assembly) {
+extern "C" int sod_set_stage_config_lds(int elems_per_workgroup, int assembly, int lds_layout) {
     static const char* names[3] = {"element-node stores", "atomic scatter", "colour-ordered scatter"};
     for (int c = 0; c < kNumStageCfgs; ++c)
-        if (kStageCfgs[c].nw == elems_per_workgroup && kStageCfgs[c].assembly == assembly) {
+        if (kStageCfgs[c].nw == elems_per_workgroup && kStageCfgs[c].assembly == assembly &&
+            kStageCfgs[c].lds == lds_layout) {
             rt().autotune = kPinned + c;
             return SOD_OK;
         }
-    return fail(SOD_ERR_ARG, "sod_set_stage_config: no configuration with %d elements per workgroup and %s",
-                elems_per_workgroup, assembly >= 0 && assembly <= 2 ? names[assembly] : "that assembly");
+    return fail(SOD_ERR_ARG,
+                "sod_set_stage_config: no configuration with %d elements per workgroup, %s and LDS layout %d",
+                elems_per_workgroup, assembly >= 0 && assembly <= 2 ? names[assembly] : "that assembly", lds_layout);
+}
+
+extern "C" int sod_set_stage_config(int elems_per_workgroup, int assembly) {
+    return sod_set_stage_config_lds(elems_per_workgroup, assembly, LDS_SWIZZLED);
 }
 
 extern "C" int sod_stage_config(const sod_time_loop* loop, int* elems_per_workgroup, int* assembly,
@@ -2091,6 +2109,22 @@ extern "C" int sod_stage_config(const sod_time_loop* loop, int* elems_per_workgr
     if (assembly) *assembly = c.assembly;
     if (tuned_mode) *tuned_mode = P->cfg_mode >= kPinned ? 3 : P->cfg_mode;
     if (stage_ms) *stage_ms = P->cfg_ms;
+    return SOD_OK;
+}
+
+extern "C" int sod_stage_lds_layout(const sod_time_loop* loop, int* lds_layout) {
+    const Prepared* P = nullptr;
+    if (loop != nullptr) {
+        P = &loop->P;
+    } else {
+        const PrepSlot* last = nullptr;
+        for (const auto& sl : g_slots)
+            if (sl.P.nelem >= 0 && (last == nullptr || sl.used > last->used)) last = &sl;
+        if (last != nullptr) P = &last->P;
+    }
+    if (P == nullptr) return fail(SOD_ERR_ARG, "sod_stage_lds_layout: no mesh has been stepped yet");
+    if (lds_layout == nullptr) return fail(SOD_ERR_ARG, "sod_stage_lds_layout: lds_layout is NULL");
+    *lds_layout = kStageCfgs[P->cfg].lds;
     return SOD_OK;
 }
 

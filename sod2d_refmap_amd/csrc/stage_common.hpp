@@ -13,6 +13,13 @@ __device__ __forceinline__ int swz(int i, int j, int k) {
     return 16 * k + ((i + k) & 3) + 4 * ((j + k) & 3);
 }
 
+// LDS slot layouts of an element's nodal values in the stage kernel
+enum { LDS_SWIZZLED = 0, LDS_LINEAR = 1 };
+template <int LAY>
+__device__ __forceinline__ int lds_slot(int i, int j, int k) {
+    return LAY == LDS_SWIZZLED ? swz(i, j, k) : i + 4 * j + 16 * k;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));

@@ -1,7 +1,7 @@
-"""Stage-kernel configurations (sod_set_autotune, sod_set_stage_config): 1, 2
-or 4 elements per workgroup with element-node stores, 4 with atomic scatter
-into node arrays, and colour-ordered scatter (one launch per element colour,
-no atomics).  Every configuration steps a curvilinear mesh with permuted
+"""Stage-kernel configurations (sod_set_autotune, sod_set_stage_config_lds):
+1, 2 or 4 elements per workgroup with element-node stores, 4 with atomic
+scatter into node arrays, colour-ordered scatter (one launch per element
+colour, no atomics), persistent wavefronts, and the linear LDS layout.  Every configuration steps a curvilinear mesh with permuted
 elements to the oracle's tolerances; the autotuner picks one of the
 candidates of its mode and reports it; the device time loop and a batched
 ensemble run with a pinned or tuned configuration and match the default
@@ -18,7 +18,8 @@ from sod2d_refmap_amd.device import numerical_params, set_numerical_params
 pytestmark = pytest.mark.gpu
 
 T = oracle.hex64_tables()
-CFGS = [(4, "pull"), (2, "pull"), (1, "pull"), (4, "atomic"), (1, "colour"), (0, "pull")]
+CFGS = [(4, "pull", "swizzled"), (2, "pull", "swizzled"), (1, "pull", "swizzled"), (4, "atomic", "swizzled"),
+        (1, "colour", "swizzled"), (0, "pull", "swizzled"), (1, "pull", "linear")]
 KEYS = ("rho", "q", "E", "u", "pr", "Tem", "eta", "csound", "machno", "mu_fluid", "mu_e", "mue_l")
 
 
@@ -47,17 +48,17 @@ def _oracle_steps(m, g, gas, s, dt, nsteps):
     return out
 
 
-@pytest.mark.parametrize("nw,assembly", CFGS, ids=[f"nw{n}_{a}" for n, a in CFGS])
-def test_each_configuration_matches_oracle(gpu, default_after, nw, assembly):
+@pytest.mark.parametrize("nw,assembly,lds", CFGS, ids=[f"nw{n}_{a}_{l}" for n, a, l in CFGS])
+def test_each_configuration_matches_oracle(gpu, default_after, nw, assembly, lds):
     from sod2d_refmap_amd.solver import PeriodicSolver
     m, gas, s = _case()
     g = mesh_geometry(m)
     dt = oracle.adapt_dt_cfl(m.connec, g["helem"], s["u"], s["csound"], 0.5, 0.5, s["mu_fluid"],
                              np.zeros((m.nelem, 64), order="F"), s["rho"])
     ref = _oracle_steps(m, g, gas, s, dt, 10)
-    _lib.set_stage_config(nw, assembly)
+    _lib.set_stage_config(nw, assembly, lds)
     sol = PeriodicSolver(m, gas, s)
-    print(f"[stage config] {nw} elements per workgroup, assembly={assembly}", flush=True)
+    print(f"[stage config] {nw} elements per workgroup, assembly={assembly}, lds={lds}", flush=True)
     for step in range(10):
         sol.rk_4_main(dt)
         if step in (0, 9):
@@ -67,7 +68,7 @@ def test_each_configuration_matches_oracle(gpu, default_after, nw, assembly):
                 err = rel_l2(out[k], ref[step][k])
                 assert err < tol, (step, k, err)
     c = _lib.stage_config()
-    assert (c["elems_per_workgroup"], c["assembly"], c["mode"]) == (nw, assembly, 3)
+    assert (c["elems_per_workgroup"], c["assembly"], c["lds"], c["mode"]) == (nw, assembly, lds, 3)
     sol.close()
 
 
@@ -96,7 +97,7 @@ def test_autotuned_step_matches_oracle(gpu, default_after, mode):
     print(f"[autotune {mode}] chose {c}", flush=True)
     assert c["mode"] == mode and c["stage_ms"] > 0.0
     fixed_order = [x for x in CFGS if x[1] != "atomic"]
-    assert (c["elems_per_workgroup"], c["assembly"]) in (CFGS if mode == 2 else fixed_order)
+    assert (c["elems_per_workgroup"], c["assembly"], c["lds"]) in (CFGS if mode == 2 else fixed_order)
     out = sol.state()
     for k in KEYS:
         err = rel_l2(out[k], ref[1][k])
