@@ -3,6 +3,7 @@
 // (smart_visc_spectral), CFL time step, kinetic-energy average, and the
 // init-time geometry routines (Jacobians, lumped mass, element lengths).
 #include <algorithm>
+#include <cstdint>
 #include <vector>
 
 #include "stage_common.hpp"
@@ -295,9 +296,11 @@ __global__ __launch_bounds__(256) void k_adapt_dt_nodes(int npoin, const double*
 // (connec, helem): every entry of both arrays enters an exact hash
 // (a 64-bit sum, mod 2^64, of a mixing function of each value's bits and
 // position, so that the order of the additions does not matter).
-// k_dt_hash adds the hash of its share into acc[0] (connec) and acc[1]
-// (helem); k_dt_cache_check compares them with the stored fingerprint, sets
-// *stale (or `force`), stores them and clears acc for the next call.
+// k_dt_hash leaves its block's two sums (connec, helem) in
+// part[2b], part[2b + 1]; k_dt_cache_check adds the blocks' sums, compares
+// them with the stored fingerprint fp[0..1], sets *stale (or `force`) and
+// stores them.  The arrays are read 16 bytes per load when their addresses
+// allow it (vec), four independent loads per thread and round.
 __device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
     // splitmix64 finaliser (Steele, Lea and Flood, OOPSLA 2014)
     x ^= x >> 30;
@@ -308,35 +311,96 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
     return x;
 }
 
+__device__ __forceinline__ unsigned long long conn_term(int v, long t) {
+    return mix64((unsigned long long)(unsigned)v ^ mix64((unsigned long long)t));
+}
+__device__ __forceinline__ unsigned long long helem_term(double v, long t) {
+    return mix64((unsigned long long)__double_as_longlong(v) ^ mix64((unsigned long long)t + 0x9e37ULL));
+}
+__device__ __forceinline__ unsigned long long conn4_terms(int4 v, long g) {
+    const long t = 4 * g;
+    return conn_term(v.x, t) + conn_term(v.y, t + 1) + conn_term(v.z, t + 2) + conn_term(v.w, t + 3);
+}
+__device__ __forceinline__ unsigned long long helem2_terms(double2 v, long g) {
+    return helem_term(v.x, 2 * g) + helem_term(v.y, 2 * g + 1);
+}
+
+constexpr int kDtHashBlocks = 1024;
+
 __global__ __launch_bounds__(256) void k_dt_hash(long n_conn, const int* __restrict__ connec, int nelem,
-                                                 const double* __restrict__ helem,
-                                                 unsigned long long* __restrict__ acc) {
+                                                 const double* __restrict__ helem, int vec,
+                                                 unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long s_a[4], s_b[4];
     unsigned long long a = 0, b = 0;
     const long stride = (long)gridDim.x * 256;
-    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n_conn; t += stride)
-        a += mix64((unsigned long long)(unsigned)connec[t] ^ mix64((unsigned long long)t));
-    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < nelem; t += stride)
-        b += mix64((unsigned long long)__double_as_longlong(helem[t]) ^ mix64((unsigned long long)t + 0x9e37ULL));
+    const long t0 = (long)blockIdx.x * 256 + threadIdx.x;
+    long done_c = 0, done_h = 0;   // entries covered by the 16-byte loops
+    if (vec) {
+        const int4* c4 = reinterpret_cast<const int4*>(connec);
+        const long n4 = n_conn / 4;
+        long g = t0;
+        for (; g + 3 * stride < n4; g += 4 * stride) {
+            const int4 v0 = c4[g], v1 = c4[g + stride], v2 = c4[g + 2 * stride], v3 = c4[g + 3 * stride];
+            a += conn4_terms(v0, g) + conn4_terms(v1, g + stride) + conn4_terms(v2, g + 2 * stride) +
+                 conn4_terms(v3, g + 3 * stride);
+        }
+        for (; g < n4; g += stride) a += conn4_terms(c4[g], g);
+        const double2* h2 = reinterpret_cast<const double2*>(helem);
+        const long n2 = nelem / 2;
+        g = t0;
+        for (; g + 3 * stride < n2; g += 4 * stride) {
+            const double2 v0 = h2[g], v1 = h2[g + stride], v2 = h2[g + 2 * stride], v3 = h2[g + 3 * stride];
+            b += helem2_terms(v0, g) + helem2_terms(v1, g + stride) + helem2_terms(v2, g + 2 * stride) +
+                 helem2_terms(v3, g + 3 * stride);
+        }
+        for (; g < n2; g += stride) b += helem2_terms(h2[g], g);
+        done_c = 4 * n4;
+        done_h = 2 * n2;
+    }
+    for (long t = done_c + t0; t < n_conn; t += stride) a += conn_term(connec[t], t);
+    for (long t = done_h + t0; t < nelem; t += stride) b += helem_term(helem[t], t);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         a += __shfl_xor(a, o, 64);
         b += __shfl_xor(b, o, 64);
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(acc, a);
-        atomicAdd(acc + 1, b);
+        s_a[threadIdx.x >> 6] = a;
+        s_b[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s_a[0] + s_a[1] + s_a[2] + s_a[3];
+        part[2 * blockIdx.x + 1] = s_b[0] + s_b[1] + s_b[2] + s_b[3];
     }
 }
 
-__global__ void k_dt_cache_check(unsigned long long* __restrict__ acc, unsigned long long* __restrict__ fp,
-                                 int* __restrict__ stale, int force) {
+__global__ __launch_bounds__(256) void k_dt_cache_check(int nblk, const unsigned long long* __restrict__ part,
+                                                        unsigned long long* __restrict__ fp,
+                                                        int* __restrict__ stale, int force) {
+    __shared__ unsigned long long s_a[4], s_b[4];
+    unsigned long long a = 0, b = 0;
+    for (int q = threadIdx.x; q < nblk; q += 256) {
+        a += part[2 * q];
+        b += part[2 * q + 1];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_a[threadIdx.x >> 6] = a;
+        s_b[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    const bool same = !force && fp[0] == acc[0] && fp[1] == acc[1];
+    a = s_a[0] + s_a[1] + s_a[2] + s_a[3];
+    b = s_b[0] + s_b[1] + s_b[2] + s_b[3];
+    const bool same = !force && fp[0] == a && fp[1] == b;
     *stale = same ? 0 : 1;
-    fp[0] = acc[0];
-    fp[1] = acc[1];
-    acc[0] = 0;
-    acc[1] = 0;
+    fp[0] = a;
+    fp[1] = b;
 }
 
 // flag = 1 if any of the n values is non-zero
@@ -712,7 +776,7 @@ struct DtNodeCache {
     const double* helem = nullptr;
     int nelem = -1, npoin = -1;
     double* hmin = nullptr;
-    unsigned long long* fp = nullptr;   // stored hashes (2), then the running sums (2)
+    unsigned long long* fp = nullptr;   // stored hashes (2), then two sums per k_dt_hash block
     int* flags = nullptr;   // [0] cache stale, [1] mu_sgs holds a non-zero value
     void release() {
         (void)hipFree(hmin);
@@ -754,8 +818,8 @@ int launch_adapt_dt_cached(int nelem, int npoin, const int* connec, const double
         SOD_HIP(hipStreamSynchronize(s));
         C.release();
         SOD_HIP(hipMalloc(&C.hmin, sizeof(double) * (size_t)std::max(1, npoin)));
-        SOD_HIP(hipMalloc(&C.fp, sizeof(unsigned long long) * 4));
-        SOD_HIP(hipMemsetAsync(C.fp, 0, sizeof(unsigned long long) * 4, s));
+        SOD_HIP(hipMalloc(&C.fp, sizeof(unsigned long long) * (2 + 2 * kDtHashBlocks)));
+        SOD_HIP(hipMemsetAsync(C.fp, 0, sizeof(unsigned long long) * 2, s));
         SOD_HIP(hipMalloc(&C.flags, sizeof(int) * 2));
         C.connec = connec;
         C.helem = helem;
@@ -767,9 +831,11 @@ int launch_adapt_dt_cached(int nelem, int npoin, const int* connec, const double
     auto* hbits = reinterpret_cast<unsigned long long*>(C.hmin);
     hipLaunchKernelGGL(k_set_double, dim3(1), dim3(64), 0, s, d_dt_bits, 100000000000000.0);
     if (nelem <= 0 || npoin <= 0) return SOD_OK;
-    hipLaunchKernelGGL(k_dt_hash, dim3(std::min(1024, grid_for((long)nelem * 64, 256))), dim3(256), 0, s,
-                       (long)nelem * 64, connec, nelem, helem, C.fp + 2);
-    hipLaunchKernelGGL(k_dt_cache_check, dim3(1), dim3(64), 0, s, C.fp + 2, C.fp, C.flags, force);
+    const int vec = (reinterpret_cast<uintptr_t>(connec) % 16 == 0 && reinterpret_cast<uintptr_t>(helem) % 16 == 0);
+    const int nblk = std::min(kDtHashBlocks, grid_for((long)nelem * 16, 256));
+    hipLaunchKernelGGL(k_dt_hash, dim3(nblk), dim3(256), 0, s, (long)nelem * 64, connec, nelem, helem, vec,
+                       C.fp + 2);
+    hipLaunchKernelGGL(k_dt_cache_check, dim3(1), dim3(256), 0, s, nblk, C.fp + 2, C.fp, C.flags, force);
     hipLaunchKernelGGL(k_fill_u64, dim3(std::min(1024, grid_for(npoin, 256))), dim3(256), 0, s, (long)npoin, hbits,
                        kInfBits, C.flags);
     hipLaunchKernelGGL(k_hmin_build, dim3(std::min(2048, grid_for((long)nelem * 64, 256))), dim3(256), 0, s, nelem,

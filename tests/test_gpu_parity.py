@@ -250,6 +250,45 @@ def test_adapt_dt_node_form_cache_and_sgs(case):
     assert run(sg0) == ref(h2, zero)
 
 
+
+def test_adapt_dt_cache_unaligned_arrays(case):
+    """The CFL cache's hash reads connec and helem 16 bytes at a time when
+    both addresses allow it, 4 or 8 bytes otherwise: with each array placed
+    at an odd 4- or 8-byte offset inside a larger buffer, dt equals the
+    oracle's bit for bit, on the first call, on a cached second call and
+    after helem is rewritten in place."""
+    name, m, g, gas, s = case
+    ne, npn = m.nelem, m.npoin
+    D = Dev()
+    conn = np.asfortranarray(m.connec).ravel(order="F").astype(np.int32)
+    cbuf = torch.zeros(conn.size + 1, dtype=torch.int32, device="cuda")
+    cbuf[1:].copy_(torch.from_numpy(conn))
+    hel = np.asfortranarray(g["helem"]).ravel(order="F")
+    hbuf = torch.zeros(hel.size + 1, dtype=torch.float64, device="cuda")
+    hbuf[1:].copy_(torch.from_numpy(hel))
+    cp, hp = cbuf[1:].data_ptr(), hbuf[1:].data_ptr()
+    assert cp % 16 != 0 and hp % 16 != 0
+    args = dict(u=D("u", s["u"]), cs=D("cs", s["csound"]), mu=D("mu", s["mu_fluid"]), r=D("r", s["rho"]))
+    zero = np.zeros((ne, 64), order="F")
+    sg0 = D("sg0", zero)
+
+    def run():
+        dt = ctypes.c_double(0.0)
+        cd = ctypes.c_double(0.5)
+        _lib.call("sod_adapt_dt_cfl", ne, npn, cp, hp, args["u"], args["cs"], 0.45, ctypes.addressof(dt),
+                  ctypes.addressof(cd), args["mu"], sg0, args["r"])
+        return dt.value
+
+    def ref(helem):
+        return oracle.adapt_dt_cfl(m.connec, helem, s["u"], s["csound"], 0.45, 0.5, s["mu_fluid"], zero, s["rho"])
+
+    for rep in range(2):
+        assert run() == ref(g["helem"]), rep
+    h2 = g["helem"].copy(order="F")
+    h2[ne // 2] *= 0.7   # one element, in place
+    hbuf[1:].copy_(torch.from_numpy(np.asfortranarray(h2).ravel(order="F")))
+    assert run() == ref(h2)
+
 def _oracle_steps(m, g, gas, s, dts):
     s = {k: v.copy(order="F") for k, v in s.items()}
     p = oracle.default_params(Rgas_inf=gas.Rgas)
