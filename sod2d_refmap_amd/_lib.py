@@ -100,21 +100,22 @@ def lib() -> ctypes.CDLL:
             raise RuntimeError(
                 f"{LIB_PATH} is not built: run `python sod2d_refmap_amd/build.py` "
                 "(or __graft_entry__.build()); there is no CPU fallback")
-        _lib = ctypes.CDLL(LIB_PATH)
+        lib_ = ctypes.CDLL(LIB_PATH)
+        missing = [name for name in _SIGS if getattr(lib_, name, None) is None]
+        if missing:
+            raise RuntimeError(f"{LIB_PATH} lacks {', '.join(missing)}: it was built from older sources; "
+                               "rebuild it with __graft_entry__.build()")
         for name, args in _SIGS.items():
-            # a symbol this build lacks raises when it is called (test_abi
-            # checks that every declared symbol is exported)
-            fn = getattr(_lib, name, None)
-            if fn is None:
-                continue
+            fn = getattr(lib_, name)
             fn.argtypes = args
             fn.restype = c_int
-        _lib.sod_last_error.restype = ctypes.c_char_p
-        _lib.sod_last_error.argtypes = []
-        _lib.sod_default_numerical_params.restype = None
-        _lib.sod_default_numerical_params.argtypes = [ctypes.POINTER(NumericalParams)]
-        _lib.sod_deviceptr.restype = c_void_p
-        _lib.sod_deviceptr.argtypes = [c_void_p]
+        lib_.sod_last_error.restype = ctypes.c_char_p
+        lib_.sod_last_error.argtypes = []
+        lib_.sod_default_numerical_params.restype = None
+        lib_.sod_default_numerical_params.argtypes = [ctypes.POINTER(NumericalParams)]
+        lib_.sod_deviceptr.restype = c_void_p
+        lib_.sod_deviceptr.argtypes = [c_void_p]
+        _lib = lib_
     return _lib
 
 

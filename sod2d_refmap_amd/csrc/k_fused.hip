@@ -2065,6 +2065,17 @@ extern "C" int sod_time_loop_destroy(sod_time_loop* L) {
     return SOD_OK;
 }
 
+// The prepared mesh a query entry point reports on: the time loop's own, or
+// (loop == NULL) the one most recently stepped through the drop-in entry
+// points; NULL when there is none.
+static const Prepared* queried_mesh(const sod_time_loop* loop) {
+    if (loop != nullptr) return &loop->P;
+    const PrepSlot* last = nullptr;
+    for (const auto& sl : g_slots)
+        if (sl.P.nelem >= 0 && (last == nullptr || sl.used > last->used)) last = &sl;
+    return last != nullptr ? &last->P : nullptr;
+}
+
 extern "C" int sod_set_autotune(int mode) {
     if (mode < 0 || mode > 2)
         return fail(SOD_ERR_ARG,
@@ -2094,15 +2105,7 @@ extern "C" int sod_set_stage_config(int elems_per_workgroup, int assembly) {
 
 extern "C" int sod_stage_config(const sod_time_loop* loop, int* elems_per_workgroup, int* assembly,
                                 int* tuned_mode, double* stage_ms) {
-    const Prepared* P = nullptr;
-    if (loop != nullptr) {
-        P = &loop->P;
-    } else {
-        const PrepSlot* last = nullptr;
-        for (const auto& sl : g_slots)
-            if (sl.P.nelem >= 0 && (last == nullptr || sl.used > last->used)) last = &sl;
-        if (last != nullptr) P = &last->P;
-    }
+    const Prepared* P = queried_mesh(loop);
     if (P == nullptr) return fail(SOD_ERR_ARG, "sod_stage_config: no mesh has been stepped yet");
     const StageCfg& c = kStageCfgs[P->cfg];
     if (elems_per_workgroup) *elems_per_workgroup = c.nw;
@@ -2113,15 +2116,7 @@ extern "C" int sod_stage_config(const sod_time_loop* loop, int* elems_per_workgr
 }
 
 extern "C" int sod_stage_lds_layout(const sod_time_loop* loop, int* lds_layout) {
-    const Prepared* P = nullptr;
-    if (loop != nullptr) {
-        P = &loop->P;
-    } else {
-        const PrepSlot* last = nullptr;
-        for (const auto& sl : g_slots)
-            if (sl.P.nelem >= 0 && (last == nullptr || sl.used > last->used)) last = &sl;
-        if (last != nullptr) P = &last->P;
-    }
+    const Prepared* P = queried_mesh(loop);
     if (P == nullptr) return fail(SOD_ERR_ARG, "sod_stage_lds_layout: no mesh has been stepped yet");
     if (lds_layout == nullptr) return fail(SOD_ERR_ARG, "sod_stage_lds_layout: lds_layout is NULL");
     *lds_layout = kStageCfgs[P->cfg].lds;
@@ -2129,15 +2124,7 @@ extern "C" int sod_stage_lds_layout(const sod_time_loop* loop, int* lds_layout) 
 }
 
 extern "C" int sod_element_colouring(const sod_time_loop* loop, int* ncolours, int* colour, int nelem) {
-    const Prepared* P = nullptr;
-    if (loop != nullptr) {
-        P = &loop->P;
-    } else {
-        const PrepSlot* last = nullptr;
-        for (const auto& sl : g_slots)
-            if (sl.P.nelem >= 0 && (last == nullptr || sl.used > last->used)) last = &sl;
-        if (last != nullptr) P = &last->P;
-    }
+    const Prepared* P = queried_mesh(loop);
     if (P == nullptr || P->color_state != 1)
         return fail(SOD_ERR_ARG, "sod_element_colouring: no coloured mesh (step one with colour-ordered assembly)");
     if (nelem != P->nelem || colour == nullptr || ncolours == nullptr)
