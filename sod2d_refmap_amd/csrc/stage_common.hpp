@@ -20,16 +20,41 @@ __device__ __forceinline__ int lds_slot(int i, int j, int k) {
     return LAY == LDS_SWIZZLED ? swz(i, j, k) : i + 4 * j + 16 * k;
 }
 
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
+// v of another lane by a DPP lane move (dpp_ctrl CTRL) in the rows that
+// ROW_MASK selects; lanes of the other rows keep v.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_lanes(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
+
+// Maximum (MAX) or minimum over the 64 lanes of a wavefront, returned to
+// every lane.  The lane moves are DPP operands of the VALU: two quad
+// permutations ([1,0,3,2], [2,3,0,1]), the half-row and row mirrors (each
+// row of 16 then holds its extremum), the broadcast of lane 15 of rows 0
+// and 2 into rows 1 and 3, then of lane 31 into rows 2 and 3; lane 63 then
+// holds the result, which one readlane makes uniform.  Shuffles through the
+// LDS unit (ds_bpermute) would make each of the six steps a dependent LDS
+// round trip.  Every lane of the wavefront must be active.  min and max are
+// exact, so the order of the comparisons does not change the value.
+template <bool MAX>
+__device__ __forceinline__ double wave_extremum(double v) {
+    auto pick = [](double a, double b) { return MAX ? fmax(a, b) : fmin(a, b); };
+    v = pick(v, dpp_lanes<0xB1, 0xF>(v));    // quad_perm [1,0,3,2]
+    v = pick(v, dpp_lanes<0x4E, 0xF>(v));    // quad_perm [2,3,0,1]
+    v = pick(v, dpp_lanes<0x141, 0xF>(v));   // row_half_mirror
+    v = pick(v, dpp_lanes<0x140, 0xF>(v));   // row_mirror
+    v = pick(v, dpp_lanes<0x142, 0xA>(v));   // row_bcast:15 into rows 1 and 3
+    v = pick(v, dpp_lanes<0x143, 0xC>(v));   // row_bcast:31 into rows 2 and 3
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
+__device__ __forceinline__ double wave_max(double v) { return wave_extremum<true>(v); }
+__device__ __forceinline__ double wave_min(double v) { return wave_extremum<false>(v); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
 // XCD b % 8).  Returns the logical block index that gives XCD x the
