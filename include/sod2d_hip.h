@@ -206,7 +206,11 @@ int sod_smart_visc_spectral(int nelem, int npoin, int npoin_w, const int *connec
  * array is shared by every call on one GPU: calls that overlap on one GPU
  * race.  Requires sod_init; a workload needing more scratch than the array
  * holds is refused with SOD_ERR_ARG (the entropy viscosity needs a fixed
- * 8.3 KiB at any mesh size). */
+ * 8.3 KiB at any mesh size).  Called while the library stream
+ * (sod_set_stream) is under graph capture, it records kernel launches only:
+ * it does not wait on return, and it refuses with SOD_ERR_ARG node-order
+ * table pointers that no earlier uncaptured call has checked (the check
+ * copies them to the host). */
 int sod_smart_visc_spectral_noalloc(int nelem, int npoin, int npoin_w, const int *connec,
                                     const int *lpoin_w, const double *Reta, const double *Ngp,
                                     const double *coord, const double *dNgp, const double *gpvol,

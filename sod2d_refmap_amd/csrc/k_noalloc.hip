@@ -5,6 +5,8 @@
 // file-scope device array instead of library-allocated memory, for callers
 // that cannot allocate device memory (code under HIP graph capture, tight
 // serving loops).  The kernels are the ones sod_smart_visc_spectral runs.
+// Captured into a graph (the library stream, sod_set_stream, under capture),
+// the call records only kernel launches.
 //
 // `scratch` below is a device symbol: in host code its name is a host-side
 // stand-in, not a device address.  Its address on each device is looked up
@@ -93,9 +95,15 @@ extern "C" int sod_smart_visc_spectral_noalloc(int nelem, int npoin, int npoin_w
         return fail(SOD_ERR_ARG, "smart_visc_spectral_noalloc: sizes");
     if (npoin_w <= 0 && rt().params.flag_normalise_entropy == 1)
         return fail(SOD_ERR_ARG, "smart_visc_spectral_noalloc: npoin_w must be positive");
+    // Under graph capture of the library stream (sod_set_stream) the call
+    // neither copies nor waits: the node-order tables must have passed an
+    // earlier uncaptured check, and the return does not synchronise.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    SOD_HIP(hipStreamIsCapturing(rt().stream, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
     // the element kernel uses the library's GLL weights and node order; the
     // caller's tables are compared with them (without remembering the result)
-    SOD_TRY(check_tables_noalloc(invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK));
+    SOD_TRY(check_tables_noalloc(invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, capturing));
     const int *dc, *dl;
     const double *dR, *dgv, *drho, *du, *dcs, *dT, *deta, *dhl, *dh;
     double *dmue, *dmuel;
@@ -118,5 +126,9 @@ extern "C" int sod_smart_visc_spectral_noalloc(int nelem, int npoin, int npoin_w
     // their sum, then the normalisation): nothing needs clearing
     launch_smart_visc(nelem, npoin, npoin_w, dc, dl, dR, dgv, gamma_gas, drho, du, dcs, dT, deta, dhl, dh, dmue,
                       dmuel, work);
+    if (capturing) {
+        SOD_HIP(hipGetLastError());
+        return SOD_OK;
+    }
     return finish_call();
 }

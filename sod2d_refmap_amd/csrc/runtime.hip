@@ -168,11 +168,19 @@ int check_tables(const double* dlxigp_ip, const int* invAtoIJK, const int* gmshA
     return SOD_OK;
 }
 
-int check_tables_noalloc(const int* invAtoIJK, const int* gmshAtoI, const int* gmshAtoJ, const int* gmshAtoK) {
+int check_tables_noalloc(const int* invAtoIJK, const int* gmshAtoI, const int* gmshAtoJ, const int* gmshAtoK,
+                         bool capturing) {
     const void* items[4] = {invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK};
     for (int k = 0; k < 4; ++k) {
         if (items[k] == nullptr) return fail(SOD_ERR_ARG, "a p=3 table argument is NULL");
         if (g_tables_ok.count({items[k], k + 1})) continue;
+        // the check copies the table to the host, which a stream under
+        // capture does not allow
+        if (capturing)
+            return fail(SOD_ERR_ARG,
+                        "node-order table %d at %p has not been checked: under stream capture the tables must be "
+                        "ones an earlier uncaptured call (any entry point taking them) has checked",
+                        k + 1, items[k]);
         SOD_TRY(check_table(items[k], k + 1));
     }
     return SOD_OK;

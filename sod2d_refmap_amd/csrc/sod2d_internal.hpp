@@ -126,7 +126,10 @@ int check_tables(const double* dlxigp_ip, const int* invAtoIJK, const int* gmshA
                  const int* gmshAtoJ, const int* gmshAtoK);
 // The node-order tables checked without remembering the result (no host
 // allocation; pointers already checked by check_tables are not read again).
-int check_tables_noalloc(const int* invAtoIJK, const int* gmshAtoI, const int* gmshAtoJ, const int* gmshAtoK);
+// With `capturing` (the library stream is under graph capture) a table not
+// checked before is refused instead of copied to the host.
+int check_tables_noalloc(const int* invAtoIJK, const int* gmshAtoI, const int* gmshAtoJ, const int* gmshAtoK,
+                         bool capturing);
 
 // Device scratch owned by the runtime, grown on demand.
 int scratch(size_t bytes, void** out, int slot);

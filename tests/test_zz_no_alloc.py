@@ -3,7 +3,12 @@ sod_smart_visc_spectral, through the C ABI: two calls in a row, the second
 on a smaller mesh, each equal bit for bit to the allocating entry point
 (same kernels, same summation order), with the entropy normalisation on
 (the branch that uses the scratch array) and off; and an oracle check of
-the first call."""
+the first call.  Captured into a HIP graph on a side stream, it records
+only kernel launches: replays equal the eager allocating call bit for bit,
+and unchecked table pointers are refused under capture."""
+import ctypes
+import warnings
+
 import numpy as np
 import pytest
 import torch
@@ -72,3 +77,56 @@ def test_noalloc_refuses_bad_sizes(gpu):
     # npoin_w > npoin: refused before any pointer is read
     args = [1, 1, 2] + [None] * 8 + [1.4] + [None] * 14
     assert _lib.lib().sod_smart_visc_spectral_noalloc(*args) == -1   # SOD_ERR_ARG
+
+
+def test_noalloc_in_captured_graph(gpu):
+    """sod_smart_visc_spectral_noalloc captured into a graph on a side stream
+    (sod_set_stream, no synchronisation on return): three replays, with eager
+    work on the default stream between them, each equal bit for bit to the
+    allocating entry point called eagerly on the same inputs and tables; under
+    capture, copies of the node-order tables at new addresses (never checked)
+    are refused with SOD_ERR_ARG."""
+    gas = cases.GasParams()
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    m, g, gas, s, Reta = _inputs(3, 7)
+    keep = []
+    dev = lambda a: keep.append(to_device(a)) or keep[-1].data_ptr()
+    ne, npn = m.nelem, m.npoin
+    ins = [dev(m.connec), dev(m.lpoin_w), dev(Reta), dev(T.Ngp), dev(m.coord), dev(T.dNgp), dev(g["gpvol"]),
+           dev(T.wgp), gas.gamma_gas, dev(s["rho"]), dev(s["u"]), dev(s["csound"]), dev(s["Tem"]), dev(s["eta"]),
+           dev(g["helem_l"]), dev(g["helem"]), dev(g["Ml"])]
+    tabs = [dev(T.invAtoIJK), dev(T.gmshAtoI), dev(T.gmshAtoJ), dev(T.gmshAtoK)]
+    nan = lambda: torch.full((64 * ne,), np.nan, dtype=torch.float64, device="cuda")
+    a_mue, a_muel, b_mue, b_muel = nan(), nan(), nan(), nan()
+    args = lambda mue, muel, t: [ne, npn, m.npoin_w, *ins, mue.data_ptr(), *t, muel.data_ptr()]
+    _lib.call("sod_smart_visc_spectral", *args(a_mue, a_muel, tabs))   # eager; checks the tables
+    torch.cuda.synchronize()
+    assert torch.isfinite(a_mue).all() and torch.isfinite(a_muel).all()
+    side = torch.cuda.Stream()
+    try:
+        _lib.call("sod_set_stream", ctypes.c_void_p(side.cuda_stream))
+        _lib.call("sod_set_sync", 0)
+        graph = torch.cuda.CUDAGraph()
+        print("[noalloc] capturing on a side stream", flush=True)
+        with torch.cuda.graph(graph, stream=side):
+            rc = _lib.lib().sod_smart_visc_spectral_noalloc(*args(b_mue, b_muel, tabs))
+        assert rc == 0, _lib.lib().sod_last_error()
+        for rep in range(3):
+            b_mue.fill_(np.nan)
+            b_muel.fill_(np.nan)
+            torch.cuda.synchronize()
+            graph.replay()
+            torch.cuda.synchronize()
+            print(f"[noalloc] replay {rep}", flush=True)
+            assert torch.equal(b_mue, a_mue) and torch.equal(b_muel, a_muel), rep
+        fresh = [dev(T.invAtoIJK), dev(T.gmshAtoI), dev(T.gmshAtoJ), dev(T.gmshAtoK)]
+        g2 = torch.cuda.CUDAGraph()
+        with warnings.catch_warnings():   # the refused call leaves the graph empty
+            warnings.simplefilter("ignore", UserWarning)
+            with torch.cuda.graph(g2, stream=side):
+                rc = _lib.lib().sod_smart_visc_spectral_noalloc(*args(b_mue, b_muel, fresh))
+        assert rc == -1 and b"stream capture" in _lib.lib().sod_last_error()
+    finally:
+        _lib.call("sod_set_stream", None)
+        _lib.call("sod_set_sync", 1)
+        torch.cuda.synchronize()
