@@ -634,7 +634,7 @@ __global__ __launch_bounds__(256) void k_to_lane_tiled(int nelem, const double* 
 // lane-layout mesh data: a block takes 16 consecutive elements, one wavefront
 // per element at a time; results are staged in LDS and written to the
 // reference-layout mu_e(nelem,64) / mue_l(nelem,64) with contiguous stores.
-__global__ __launch_bounds__(256) void k_smart_visc_lane(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_smart_visc_lane(
     int nelem, int npoin, const int* __restrict__ conn_l, const double* __restrict__ gpv_l,
     const double* __restrict__ helem_l, const double* __restrict__ helem, const double* __restrict__ Reta,
     double gamma_gas, const double* __restrict__ rho, const double* __restrict__ u,
@@ -661,11 +661,21 @@ __global__ __launch_bounds__(256) void k_smart_visc_lane(
     }
     __syncthreads();
     const int a = s_l2n[l];
-    // the wavefront's 4 elements unrolled, so that their gathers are issued
-    // together (13% less time than one element after the other)
+    // The wavefront's 4 elements unrolled, so that their gathers are issued
+    // together (13% less time than one element after the other): per
+    // element the extrema over its 64 lanes (min/max J, max |u|, max c,
+    // max beta, uniform) and this lane's R1.  Elements beyond nelem keep
+    // placeholders that are never stored.
+    double ex[4][4], bet[4], R1v[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int el = w + 4 * r, e = e0 + el;
+        ex[r][0] = 1.0;
+        ex[r][1] = 1.0;
+        ex[r][2] = 0.0;
+        ex[r][3] = 1.0;
+        bet[r] = 0.0;
+        R1v[r] = 0.0;
         if (e < nelem) {   // uniform per wavefront
             const double norm = normalise ? d_norm[elem_inst ? elem_inst[e] : 0] : 1.0;
             const size_t eb = (size_t)e * 64 + l;
@@ -673,34 +683,61 @@ __global__ __launch_bounds__(256) void k_smart_visc_lane(
             const double J = gpv_l[eb] / s_w[l];
             const double u0 = u[n], u1 = u[n + np], u2 = u[n + 2 * np];
             const double vmag = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
-            const double minJe = wave_min(fmin(1000000.0, J));
-            const double maxJe = wave_max(fmax(0.0, J));
-            const double maxV = wave_max(vmag);
-            const double maxC = wave_max(csound[n]);
-            const double M = maxV / maxC;
-            double M15 = M * M;
-            M15 = M15 * M15 * M;
-            M15 = M15 * M15 * M15;
-            const double ceM = fmax(tanh(M15 * 3.14159265358979323846), ce);
-            const double rr = minJe / maxJe;
-            double ced = fmax(1.0 - rr * rr, ce);
-            ced = fmax(ced, ceM);
+            ex[r][0] = wave_min(fmin(1000000.0, J));
+            ex[r][1] = wave_max(fmax(0.0, J));
+            ex[r][2] = wave_max(vmag);
+            ex[r][3] = wave_max(csound[n]);
             const double aux3 = sqrt(Rgas * gamma_gas * Tem[n]);
-            const double betae = wave_max((rho[n] * helem[e]) * (cmax / 3.0) * (vmag + aux3));
-            const double R1 = rho[n] * fabs(Reta[n]) / norm;
+            bet[r] = wave_max((rho[n] * helem[e]) * (cmax / 3.0) * (vmag + aux3));
+            R1v[r] = rho[n] * fabs(Reta[n]) / norm;
+        }
+    }
+    // The limiter ced of element r is the same in its 64 lanes: it is formed
+    // once for the 4 elements, element r in the lanes l with l % 4 == r (the
+    // same operations on the same values), and read back from lane r.
+    const int rl = l & 3;
+    double xl[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) xl[k] = rl == 0 ? ex[0][k] : rl == 1 ? ex[1][k] : rl == 2 ? ex[2][k] : ex[3][k];
+    const double M = xl[2] / xl[3];
+    double M15 = M * M;
+    M15 = M15 * M15 * M;
+    M15 = M15 * M15 * M15;
+    const double ceM = fmax(tanh(M15 * 3.14159265358979323846), ce);
+    const double rr = xl[0] / xl[1];
+    double ced_l = fmax(1.0 - rr * rr, ce);
+    ced_l = fmax(ced_l, ceM);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int el = w + 4 * r, e = e0 + el;
+        if (e < nelem) {
+            const long long cb = __double_as_longlong(ced_l);
+            const int lo = __builtin_amdgcn_readlane((int)cb, r), hi = __builtin_amdgcn_readlane((int)(cb >> 32), r);
+            const double ced = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
             const double hl = s_me[el][a];
-            const double Ve = ced * R1 * (hl * hl);
-            s_ml[el][a] = cglob * fmin(Ve, betae);
+            const double Ve = ced * R1v[r] * (hl * hl);
+            s_ml[el][a] = cglob * fmin(Ve, bet[r]);
         }
     }
     __syncthreads();
     // 27-point (1/4,1/2,1/4)^3 filter over geometric positions; the index
     // table convertIJK(0:5) = {3,1,3,4,2,3} (mod_filters.f90:42-51) maps a
-    // geometric position to a sod2d 1-D index, with its end entries as built
+    // geometric position to a sod2d 1-D index, with its end entries as built.
+    // Each tap's slot is formed once and applied to the wavefront's 4
+    // elements; every element's sum keeps the taps' order.
     const int conv[6] = {3, 1, 3, 4, 2, 3};
     const int gi = (l & 3) + 1, gj = ((l >> 2) & 3) + 1, gk = (l >> 4) + 1;
     const double w3[3] = {0.25, 0.5, 0.25};
     const int so = s_l2n[(conv[gi] - 1) + 4 * (conv[gj] - 1) + 16 * (conv[gk] - 1)];
+    int tap[27];
+#pragma unroll
+    for (int ll = -1; ll <= 1; ++ll)
+#pragma unroll
+        for (int mm = -1; mm <= 1; ++mm)
+#pragma unroll
+            for (int nn = -1; nn <= 1; ++nn)
+                tap[9 * (ll + 1) + 3 * (mm + 1) + nn + 1] =
+                    s_l2n[(conv[gi + ll] - 1) + 4 * (conv[gj + mm] - 1) + 16 * (conv[gk + nn] - 1)];
     for (int r = 0; r < 4; ++r) {
         const int el = w + 4 * r, e = e0 + el;
         if (e >= nelem) break;
@@ -710,10 +747,8 @@ __global__ __launch_bounds__(256) void k_smart_visc_lane(
 #pragma unroll
             for (int mm = -1; mm <= 1; ++mm)
 #pragma unroll
-                for (int nn = -1; nn <= 1; ++nn) {
-                    const int b = s_l2n[(conv[gi + ll] - 1) + 4 * (conv[gj + mm] - 1) + 16 * (conv[gk + nn] - 1)];
-                    acc = acc + w3[ll + 1] * w3[mm + 1] * w3[nn + 1] * s_ml[el][b];
-                }
+                for (int nn = -1; nn <= 1; ++nn)
+                    acc = acc + w3[ll + 1] * w3[mm + 1] * w3[nn + 1] * s_ml[el][tap[9 * (ll + 1) + 3 * (mm + 1) + nn + 1]];
         s_me[el][so] = acc;
     }
     __syncthreads();
