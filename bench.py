@@ -140,6 +140,58 @@ def measured_flops_per_wave(kernel: str, n: int):
         return None
 
 
+def curvilinear_rate(steps: int, n: int):
+    """The randomised curvilinear workload: n^3 HEX64 elements in a periodic
+    box with the smooth node mapping x_i += 0.05 h sin(2 pi x_j/L)
+    sin(2 pi x_k/L), elements numbered by a seeded random permutation,
+    Taylor-Green plus seeded Fourier modes (|k| <= 8, amplitude 1e-2),
+    Re 3200, Mach 0.2.  Two untimed and `steps` timed eager steps (drop-in
+    rk_4_main + adapt_dt_cfl, as for the main workload): GDOF/s per RK4 stage,
+    and the stage kernel's and the whole stage's algorithmic bytes against
+    the HBM peak."""
+    import torch
+    from sod2d_refmap_amd import _lib, cases, mesh
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    t0 = time.perf_counter()
+    m = mesh.periodic_box(n, curved_amp=0.05, perm_seed=1)
+    gas = cases.GasParams(Re=3200.0, Mach=0.2)
+    sol = PeriodicSolver(m, gas, cases.placeholder_state(m, gas), cfl_conv=0.5, cfl_diff=0.5)
+    cases.tgv_state_on_device(sol, m, gas, modes_seed=2)
+    log(f"curvilinear {n}^3: nelem={m.nelem} npoin_w={m.npoin_w} ({time.perf_counter() - t0:.1f}s)")
+    sol.adapt_dt()
+    for _ in range(2):
+        sol.rk_4_main(sol.dt)
+        sol.adapt_dt()
+    torch.cuda.synchronize()
+    _lib.call("sod_profile_enable", 1)
+    _lib.call("sod_profile_reset")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sol.rk_4_main(sol.dt)
+        sol.adapt_dt()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    _lib.call("sod_profile_enable", 0)
+    ms, cnt = kernel_ms("stage_rhs")
+    Ne, Nn = m.nelem, m.npoin_w
+    alg = Ne * 5888 + Nn * 96
+    stage_bytes = Ne * 5888 + Nn * 272
+    stages = 4 * steps
+    kern_gbs = alg / (ms / cnt / 1e3) / 1e9
+    stage_gbs = stage_bytes / (el / stages) / 1e9
+    log(f"curvilinear {n}^3: {steps} steps in {el:.3f}s")
+    res = {"workload": f"curvilinear_permuted_fourier_hex64_{n}^3_re3200_mach0.2", "elements": Ne,
+           "working_nodes": Nn, "steps": steps, "ms_per_step": round(el / steps * 1e3, 4),
+           "GDOF_s": round(5.0 * Nn * stages / el / 1e9, 4),
+           "stage_rhs_ms": round(ms / cnt, 4), "stage_rhs_frac_of_hbm_peak": round(kern_gbs / HBM_PEAK_GBS, 4),
+           "stage_frac_of_hbm_peak": round(stage_gbs / HBM_PEAK_GBS, 4)}
+    sol.close()
+    del sol
+    torch.cuda.empty_cache()
+    return res
+
+
 def adjoint_ratio(sol, m, reps: int = 5):
     """Time of one adjoint (VJP) of the full stage RHS over one forward RHS,
     both through the C ABI on the solver's current state (HIP events on the
@@ -223,6 +275,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--curvilinear-n", type=int, default=96,
+                    help="elements per direction of the randomised curvilinear workload (0: skip it)")
     ap.add_argument("--batch", type=int, default=48,
                     help="also time the seeded ensemble of this many boxes (0: skip)")
     ap.add_argument("--stage-config", default="",
@@ -366,6 +420,9 @@ def main():
         out["batch"] = batch_speedup(args.batch, max(2, K // 2))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sol, m, gas, s, args.cpu_seconds)
+    if args.curvilinear_n > 0:
+        sol.close()
+        out["curvilinear"] = curvilinear_rate(max(3, K // 4), args.curvilinear_n)
     print(json.dumps(out), flush=True)
 
 

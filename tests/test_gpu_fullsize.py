@@ -19,8 +19,8 @@ with the mesh:
   adjoint against central differences with the entropy viscosity frozen, and
   the checkpointed gradient of the full step against one 200-step segment.
 
-The initial states follow sod2d_refmap_amd.cases.tgv_initial_state; the
-element-wise part is evaluated on the device from the solver's coordinates
+The initial states are sod2d_refmap_amd.cases.tgv_state_on_device:
+tgv_initial_state evaluated on the device from the solver's coordinates
 (the host version takes minutes at 96^3)."""
 import numpy as np
 import pytest
@@ -29,7 +29,6 @@ import torch
 import oracle
 from _util import rel_l2, stage_fields
 from sod2d_refmap_amd import _lib, cases, mesh as pmesh
-from sod2d_refmap_amd.mesh import hex64_tables
 from sod2d_refmap_amd.device import numerical_params, ptr, set_numerical_params
 
 pytestmark = pytest.mark.gpu
@@ -42,79 +41,6 @@ CASES = {
 INTERIOR = np.nonzero((T.gmshAtoI >= 3) & (T.gmshAtoJ >= 3) & (T.gmshAtoK >= 3))[0]   # local volume nodes
 
 
-def _placeholder(m, gas):
-    npn, ne = m.npoin, m.nelem
-    one, zero = np.ones(npn), np.zeros(npn)
-    z3, ze = np.zeros((npn, 3), order="F"), np.zeros((ne, 64), order="F")
-    return dict(rho=one, u=z3, q=z3, pr=one, E=one, Tem=one, e_int=one, eta=zero, csound=one, machno=zero,
-                mu_fluid=zero, mu_factor=np.full(npn, gas.mu_factor), mu_e=ze, mu_sgs=ze, mue_l=ze)
-
-
-def _fourier(m, seed, amp=1e-2, kmax=8.0):
-    """cases.fourier_velocity_perturbation with the separable sums as matrix
-    products (npoin, 3)."""
-    rng = np.random.default_rng(seed)
-    km = int(np.floor(kmax))
-    ks = np.arange(-km, km + 1)
-    K1, K2, K3 = np.meshgrid(ks, ks, ks, indexing="ij")
-    mask = (K1 ** 2 + K2 ** 2 + K3 ** 2 <= kmax ** 2) & ~((K1 == 0) & (K2 == 0) & (K3 == 0))
-    r = hex64_tables().gll_roots
-    axes = []
-    for nel, Ld in zip(m.n, m.L):
-        idx = np.arange(3 * nel + 1)
-        h = Ld / nel
-        e = np.minimum(idx // 3, nel - 1)
-        xs = e * h + (1.0 + r[idx - 3 * e]) * (0.5 * h)
-        axes.append(np.exp(1j * np.outer(ks, 2 * np.pi * xs / Ld)))
-    nk, lat = len(ks), m.node_lattice
-    out = np.zeros((m.npoin, 3))
-    for c in range(3):
-        a = (rng.standard_normal(K1.shape) + 1j * rng.standard_normal(K1.shape)) * amp
-        a = np.where(mask, a, 0.0)
-        M = [ax.shape[1] for ax in axes]
-        t1 = a.reshape(nk, nk * nk).T @ axes[0]                                  # (b c, I)
-        t2 = t1.reshape(nk, nk * M[0]).T @ axes[1]                               # (c I, J)
-        f = (t2.reshape(nk, M[0] * M[1]).T @ axes[2]).real.reshape(M[0], M[1], M[2])
-        out[:, c] = f[lat[:, 0], lat[:, 1], lat[:, 2]]
-    return out
-
-
-def _device_state(sol, m, gas, modes_seed):
-    """Level-2 state of tgv_initial_state written into the solver's arrays."""
-    npn = m.npoin
-    xyz = sol.coord.view(3, npn)
-    x, y, z = (xyz[d].float() for d in range(3))          # coordinates read as real(4) (TGVSolver.f90:121-123)
-    V0, L = float(np.float32(gas.V0)), float(np.float32(gas.L))
-    u = torch.zeros(3, npn, dtype=torch.float64, device="cuda")
-    u[0] = (V0 * torch.sin(x / L) * torch.cos(y / L) * torch.cos(z / L)).double()
-    u[1] = (-V0 * torch.cos(x / L) * torch.sin(y / L) * torch.cos(z / L)).double()
-    xd, yd, zd = x.double(), y.double(), z.double()
-    Lf = float(gas.L)
-    pr = gas.po + ((gas.V0 * gas.V0) / 16.0) * (torch.cos(2.0 * xd / Lf) + torch.cos(2.0 * yd / Lf)) * (
-        torch.cos(2.0 * zd / Lf) + 2.0)
-    if modes_seed is not None:
-        u += torch.from_numpy(np.ascontiguousarray(_fourier(m, modes_seed).T)).cuda()
-    rho = pr / gas.Rgas / gas.to
-    gm = gas.gamma_gas
-    e_int = pr / (rho * (gm - 1.0))
-    Tem = pr / (rho * gas.Rgas)
-    E = rho * (0.5 * (u * u).sum(0) + e_int)
-    cs = torch.sqrt(gm * pr / rho)
-    eta = (rho / (gm - 1.0)) * torch.log(pr / rho ** gm)
-    mu = sol.mu_factor * 0.000001458 * Tem ** 1.5 / (Tem + 110.4)
-    sol.level("rho").copy_(rho)
-    sol.level("u").copy_(u.reshape(-1))
-    sol.level("q").copy_((rho[None, :] * u).reshape(-1))
-    sol.level("pr").copy_(pr)
-    sol.level("E").copy_(E)
-    sol.level("Tem").copy_(Tem)
-    sol.level("e_int").copy_(e_int)
-    sol.level("eta").copy_(eta)
-    sol.csound.copy_(cs)
-    sol.mu_fluid.copy_(mu)
-    torch.cuda.synchronize()
-
-
 @pytest.fixture(scope="module", params=list(CASES))
 def big(request, gpu):
     from sod2d_refmap_amd.solver import PeriodicSolver
@@ -124,8 +50,8 @@ def big(request, gpu):
     gas = cases.GasParams(Re=kw["Re"], Mach=kw["Mach"])
     set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
     print(f"[fullsize] {request.param}: {m.nelem} elements, {m.npoin_w} working nodes; solver set-up", flush=True)
-    sol = PeriodicSolver(m, gas, _placeholder(m, gas))
-    _device_state(sol, m, gas, kw["modes_seed"])
+    sol = PeriodicSolver(m, gas, cases.placeholder_state(m, gas))
+    cases.tgv_state_on_device(sol, m, gas, kw["modes_seed"])
     print(f"[fullsize] {request.param}: state ready", flush=True)
     yield request.param, m, gas, sol
     sol.close()
@@ -138,7 +64,7 @@ def test_rhs_of_sampled_elements_matches_oracle(big):
     ne, npn = m.nelem, m.npoin
     ts, saved = _snapshot(sol)
     if CASES[name]["modes_seed"] is None:
-        _device_state(sol, m, gas, 3)
+        cases.tgv_state_on_device(sol, m, gas, 3)
     gen = torch.Generator(device="cuda").manual_seed(17)
     mu_e = torch.rand(ne * 64, dtype=torch.float64, device="cuda", generator=gen) * 2e-3
     z = lambda k: torch.zeros(k, dtype=torch.float64, device="cuda")
