@@ -412,6 +412,13 @@ int sod_set_stage_config_lds(int elems_per_workgroup, int assembly, int lds_layo
  * stage configuration of loop's mesh (NULL: the mesh most recently stepped
  * through sod_rk_4_main). */
 int sod_stage_lds_layout(const sod_time_loop *loop, int *lds_layout);
+/* No reference counterpart: 1 when the stage kernel of loop's mesh (NULL:
+ * the mesh most recently stepped through sod_rk_4_main) takes its elements
+ * in a locality order of its own -- breadth first over elements sharing
+ * nodes, built at mesh preparation when fewer than half of the caller's
+ * consecutive elements share a node -- else 0.  The order does not change
+ * the results. */
+int sod_stage_element_order(const sod_time_loop *loop, int *reordered);
 /* The stage configuration of loop's mesh (loop NULL: the mesh most recently
  * stepped through sod_rk_4_main): elements per workgroup, assembly (0-2 as
  * above), the mode it was chosen under (0-2, or 3 when pinned) and the

@@ -85,6 +85,7 @@ _SIGS = {
     "sod_set_stage_config": [c_int, c_int],
     "sod_set_stage_config_lds": [c_int, c_int, c_int],
     "sod_stage_lds_layout": [c_void_p, ctypes.POINTER(c_int)],
+    "sod_stage_element_order": [c_void_p, ctypes.POINTER(c_int)],
     "sod_element_colouring": [c_void_p, ctypes.POINTER(c_int), P, c_int],
     "sod_stage_config": [c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                          ctypes.POINTER(c_double)],
@@ -159,9 +160,11 @@ def stage_config(loop=None) -> dict:
     nw, at, md, ms, lay = c_int(), c_int(), c_int(), c_double(), c_int()
     call("sod_stage_config", loop, ctypes.byref(nw), ctypes.byref(at), ctypes.byref(md), ctypes.byref(ms))
     call("sod_stage_lds_layout", loop, ctypes.byref(lay))
+    reo = c_int()
+    call("sod_stage_element_order", loop, ctypes.byref(reo))
     return {"elems_per_workgroup": nw.value, "assembly": _ASSEMBLY_NAMES.get(at.value, str(at.value)),
             "atomic_scatter": at.value == 1, "lds": _LDS_NAMES.get(lay.value, str(lay.value)), "mode": md.value,
-            "stage_ms": ms.value}
+            "stage_ms": ms.value, "locality_order": bool(reo.value)}
 
 
 def exported_symbols() -> list[str]:

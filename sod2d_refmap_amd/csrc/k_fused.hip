@@ -95,7 +95,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
     const int lb = NW == 1 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     if (lb * NW + w >= nelem) return;   // uniform per wavefront; only wave-level sync below
     const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
-    int e = ASM == ASM_COLOR ? elist[blockIdx.x] : lb * NW + w;
+    // ASM_COLOR: this colour's element list; otherwise the prepared mesh's
+    // traversal order when it has one (elist non-null), else lb * NW + w
+    int e = lb * NW + w;
+    if (ASM == ASM_COLOR)
+        e = elist[blockIdx.x];
+    else if (!PERSIST && elist != nullptr)
+        e = elist[e];
     int n = conn[(size_t)e * 64 + l];
     for (;;) {   // one element, or (PERSIST) every G-th element
     const size_t eb = (size_t)e * 64;
@@ -842,6 +848,9 @@ struct Prepared {
     int* color_elems = nullptr;
     std::vector<int> color_off;
     int color_state = 0;   // 0 not built, 1 built, 2 the mesh cannot be coloured
+    // the stage kernel's traversal order (locality_order), or null: the
+    // caller's element order
+    int* eorder = nullptr;
     void release() {
         (void)hipFree(conn);
         (void)hipFree(He);
@@ -860,6 +869,7 @@ struct Prepared {
         (void)hipFree(fpbuf);
         (void)hipFree(rn);
         (void)hipFree(color_elems);
+        (void)hipFree(eorder);
         *this = Prepared();
     }
 };
@@ -954,6 +964,51 @@ static int check_mesh_size(int nelem) {
     return SOD_OK;
 }
 
+// A traversal order for the stage kernel when the caller's element numbering
+// keeps no locality (fewer than half of the consecutive elements share a
+// node, as after a random permutation): breadth first over the elements that
+// share nodes, from element 0 and then from each element not yet reached, so
+// that elements processed close in time gather many of the same nodes while
+// those are still in the caches.  From the lane-layout connectivity cl and
+// its inverse (off, pos) alone.  Returns false, leaving the caller's order,
+// when that order keeps its locality.  The order changes which element a
+// workgroup takes, not where its residuals are stored: results are the same
+// bits either way.
+static bool locality_order(const std::vector<int>& cl, const std::vector<int>& off, const std::vector<int>& pos,
+                           int nelem, int npoin, std::vector<int>& order) {
+    if (nelem < 2) return false;
+    std::vector<int> stamp((size_t)npoin, -1);
+    long shared = 0;
+    for (int e = 0; e + 1 < nelem; ++e) {
+        const int* a = cl.data() + (size_t)e * 64;
+        for (int l = 0; l < 64; ++l) stamp[a[l]] = e;
+        bool hit = false;
+        for (int l = 0; l < 64 && !hit; ++l) hit = stamp[a[64 + l]] == e;
+        shared += hit;
+    }
+    if (2 * shared >= (long)(nelem - 1)) return false;
+    order.clear();
+    order.reserve((size_t)nelem);
+    std::vector<char> seen((size_t)nelem, 0);
+    for (int s0 = 0; s0 < nelem; ++s0) {
+        if (seen[s0]) continue;
+        seen[s0] = 1;
+        order.push_back(s0);
+        for (size_t h = order.size() - 1; h < order.size(); ++h) {
+            const int* a = cl.data() + (size_t)order[h] * 64;
+            for (int l = 0; l < 64; ++l)
+                for (int q = off[a[l]]; q < off[a[l] + 1]; ++q) {
+                    const int f = pos[q] >> 6;
+                    if (!seen[f]) {
+                        seen[f] = 1;
+                        order.push_back(f);
+                    }
+                }
+        }
+    }
+    return true;
+}
+
 static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* connec, const int* lpoin_w,
                    const double* He, const double* gpvol, const double* Ml) {
     SOD_TRY(check_mesh_size(nelem));
@@ -1012,6 +1067,11 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
         for (size_t n = 0; n < np; ++n) off[n + 1] += off[n];
         std::vector<int> fill(off.begin(), off.end() - 1);
         for (size_t t = 0; t < nl; ++t) pos[fill[cl[t]]++] = (int)t;
+        std::vector<int> order;
+        if (locality_order(cl, off, pos, nelem, npoin, order)) {
+            SOD_HIP(hipMalloc(&P.eorder, sizeof(int) * ne));
+            SOD_HIP(hipMemcpy(P.eorder, order.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+        }
         SOD_HIP(hipMalloc(&P.inv_off, sizeof(int) * (np + 1)));
         SOD_HIP(hipMalloc(&P.inv_pos, sizeof(int) * (nl + 1)));
         SOD_HIP(hipMalloc(&P.relem, sizeof(double) * (nl * NRES + 2)));
@@ -1284,7 +1344,7 @@ static void launch_rhs_t(hipStream_t s, int nelem, const Prepared& P, const doub
     } else {
         hipLaunchKernelGGL((k_stage_rhs<HS, NW, AS, TH, false, LAY>), dim3(grid_for(nelem, NW)), dim3(64 * NW), 0, s,
                            nelem,
-                           (const int*)nullptr, P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2], in[3],
+                           (const int*)P.eorder, P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2], in[3],
                            in[4], in[5], AS == ASM_PULL ? P.relem : P.rn, np, sp);
     }
 }
@@ -2155,6 +2215,14 @@ extern "C" int sod_stage_lds_layout(const sod_time_loop* loop, int* lds_layout) 
     if (P == nullptr) return fail(SOD_ERR_ARG, "sod_stage_lds_layout: no mesh has been stepped yet");
     if (lds_layout == nullptr) return fail(SOD_ERR_ARG, "sod_stage_lds_layout: lds_layout is NULL");
     *lds_layout = kStageCfgs[P->cfg].lds;
+    return SOD_OK;
+}
+
+extern "C" int sod_stage_element_order(const sod_time_loop* loop, int* reordered) {
+    const Prepared* P = queried_mesh(loop);
+    if (P == nullptr) return fail(SOD_ERR_ARG, "sod_stage_element_order: no mesh has been stepped yet");
+    if (reordered == nullptr) return fail(SOD_ERR_ARG, "sod_stage_element_order: reordered is NULL");
+    *reordered = P->eorder != nullptr ? 1 : 0;
     return SOD_OK;
 }
 

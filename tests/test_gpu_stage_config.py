@@ -241,3 +241,43 @@ def test_colouring_is_valid(gpu, default_after, kw):
     print(f"[colouring] {m.nelem} elements, {nc.value} colours", flush=True)
     for k in KEYS:
         assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def _consecutive_sharing(m):
+    conn = np.asarray(m.connec)
+    return np.mean([len(np.intersect1d(conn[e], conn[e + 1])) > 0 for e in range(m.nelem - 1)])
+
+
+def test_locality_order_of_permuted_elements(gpu, default_after):
+    """Elements numbered by a random permutation (6^3 curved box: few
+    consecutive elements share a node) are taken by the stage kernel in the
+    library's breadth-first order; steps match the oracle after 1 and 10 steps.
+    A box in its natural order keeps the caller's order."""
+    from sod2d_refmap_amd.solver import PeriodicSolver
+    m = pmesh.periodic_box(6, curved_amp=0.05, perm_seed=21)
+    assert _consecutive_sharing(m) < 0.5
+    gas = cases.GasParams()
+    s = cases.tgv_initial_state(m, gas, perturb_seed=4, perturb_amp=5e-2)
+    set_numerical_params(numerical_params(Rgas_inf=gas.Rgas))
+    g = mesh_geometry(m)
+    dt = oracle.adapt_dt_cfl(m.connec, g["helem"], s["u"], s["csound"], 0.5, 0.5, s["mu_fluid"],
+                             np.zeros((m.nelem, 64), order="F"), s["rho"])
+    ref = _oracle_steps(m, g, gas, s, dt, 10)
+    sol = PeriodicSolver(m, gas, s)
+    for step in range(10):
+        sol.rk_4_main(dt)
+        if step in (0, 9):
+            out = sol.state()
+            tol = 1e-12 if step == 0 else 1e-10
+            for k in KEYS:
+                err = rel_l2(out[k], ref[step][k])
+                assert err < tol, (step, k, err)
+    assert _lib.stage_config()["locality_order"] is True
+    print(f"[stage config] permuted 6^3: locality order, {m.nelem} elements", flush=True)
+    sol.close()
+    m2 = pmesh.periodic_box(6, curved_amp=0.05)
+    assert _consecutive_sharing(m2) >= 0.5
+    sol2 = PeriodicSolver(m2, gas, cases.tgv_initial_state(m2, gas))
+    sol2.rk_4_main(dt)
+    assert _lib.stage_config()["locality_order"] is False
+    sol2.close()
