@@ -174,6 +174,11 @@ def curvilinear_rate(steps: int, n: int):
     el = time.perf_counter() - t0
     _lib.call("sod_profile_enable", 0)
     ms, cnt = kernel_ms("stage_rhs")
+    per_step = {}
+    for name in ("stage_rhs", "stage_update", "smart_visc_fused"):
+        kms, kc = kernel_ms(name)
+        if kc:
+            per_step[name] = round(kms / steps, 4)
     Ne, Nn = m.nelem, m.npoin_w
     alg = Ne * 5888 + Nn * 96
     stage_bytes = Ne * 5888 + Nn * 272
@@ -184,6 +189,7 @@ def curvilinear_rate(steps: int, n: int):
     res = {"workload": f"curvilinear_permuted_fourier_hex64_{n}^3_re3200_mach0.2", "elements": Ne,
            "working_nodes": Nn, "steps": steps, "ms_per_step": round(el / steps * 1e3, 4),
            "GDOF_s": round(5.0 * Nn * stages / el / 1e9, 4),
+           "kernels_ms_per_step": per_step,
            "stage_rhs_ms": round(ms / cnt, 4), "stage_rhs_frac_of_hbm_peak": round(kern_gbs / HBM_PEAK_GBS, 4),
            "stage_frac_of_hbm_peak": round(stage_gbs / HBM_PEAK_GBS, 4)}
     sol.close()

@@ -60,7 +60,7 @@ constexpr int BLK = 256;
 // mufac is mu_factor with the Sutherland gate on (P.suth) and the caller's
 // mu_fluid otherwise.
 // PERSIST (one wavefront per workgroup, element-node stores): the wavefront
-// of workgroup b takes elements b, b + G, b + 2G, ... (G = grid size) and
+// of workgroup b takes traversal positions b, b + G, b + 2G, ... (G = grid size) and
 // loads the next element's node ids after the current element's own loads,
 // so that an element waits for one round of loads (its gathers, issued with
 // its geometry) instead of two (the node ids, then the gathers).
@@ -95,13 +95,16 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
     const int lb = NW == 1 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     if (lb * NW + w >= nelem) return;   // uniform per wavefront; only wave-level sync below
     const int i = l & 3, j = (l >> 2) & 3, k = l >> 4;
-    // ASM_COLOR: this colour's element list; otherwise the prepared mesh's
-    // traversal order when it has one (elist non-null), else lb * NW + w
-    int e = lb * NW + w;
+    // tp: the traversal position, where the element-node residuals are
+    // stored; e: the element, elist[tp] when the prepared mesh has a
+    // traversal order (elist non-null), else tp.  ASM_COLOR: this colour's
+    // element list.
+    int tp = lb * NW + w;
+    int e = tp;
     if (ASM == ASM_COLOR)
         e = elist[blockIdx.x];
-    else if (!PERSIST && elist != nullptr)
-        e = elist[e];
+    else if (elist != nullptr)
+        e = elist[tp];
     int n = conn[(size_t)e * 64 + l];
     for (;;) {   // one element, or (PERSIST) every G-th element
     const size_t eb = (size_t)e * 64;
@@ -133,9 +136,12 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
     const double mu_s = HAS_SGS ? musgs[eb + l] : 0.0;
     // the next element's node ids, issued after this element's loads so
     // that waiting for those does not wait for them
-    const int e_next = e + (int)gridDim.x;
-    int n_next = 0;
-    if (PERSIST && e_next < nelem) n_next = conn[(size_t)e_next * 64 + l];
+    const int tp_next = tp + (int)gridDim.x;
+    int e_next = 0, n_next = 0;
+    if (PERSIST && tp_next < nelem) {
+        e_next = elist != nullptr ? elist[tp_next] : tp_next;
+        n_next = conn[(size_t)e_next * 64 + l];
+    }
 
     double(*S)[64] = sh[w];
     const int me = lds_slot<LAY>(i, j, k);
@@ -301,12 +307,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
         // The residuals of this element at this node, stored in element-node
         // order; k_stage_update and k_assemble_nodes sum each node's
         // contributions (pull assembly: no atomics, a fixed summation order).
-        double2* o = reinterpret_cast<double2*>(relem + (eb + l) * NRES);
+        double2* o = reinterpret_cast<double2*>(relem + ((size_t)tp * 64 + l) * NRES);
         o[0] = make_double2(res[0], res[1]);
         o[1] = make_double2(res[2], res[3]);
         o[2] = make_double2(res[4], res[5]);
     }
-    if (!PERSIST || e_next >= nelem) break;
+    if (!PERSIST || tp_next >= nelem) break;
+    tp = tp_next;
     e = e_next;
     n = n_next;
     wave_sync();   // this element's LDS reads before the next element's writes
@@ -831,7 +838,11 @@ struct Prepared {
     int* inv_off = nullptr;   // npoin+1: node n's positions are inv_pos[inv_off[n] .. inv_off[n+1])
     int* inv_pos = nullptr;   // nelem*64 positions e*64 + lane, ascending per node
     int* woff = nullptr;      // npoin_w+1: the offsets of the working node at lpoin_w position it
-    int* wpos = nullptr;      // its positions: inv_pos itself when lpoin_w increases, else a copy
+    int* wpos = nullptr;      // its positions: inv_pos_r itself when lpoin_w increases, else a copy
+    // the positions the stage kernel stores at: r*64 + lane for the element
+    // at traversal position r (eorder), in inv_pos's order; inv_pos itself
+    // when there is no traversal order
+    int* inv_pos_r = nullptr;
     int* flag = nullptr;
     double* dscal = nullptr;  // device scalars: [0] dt of the step
     double* fpbuf = nullptr;
@@ -861,7 +872,8 @@ struct Prepared {
         (void)hipFree(work);
         (void)hipFree(relem);
         (void)hipFree(inv_off);
-        if (wpos != inv_pos) (void)hipFree(wpos);
+        if (wpos != inv_pos && wpos != inv_pos_r) (void)hipFree(wpos);
+        if (inv_pos_r != inv_pos) (void)hipFree(inv_pos_r);
         (void)hipFree(inv_pos);
         (void)hipFree(woff);
         (void)hipFree(flag);
@@ -971,9 +983,10 @@ static int check_mesh_size(int nelem) {
 // that elements processed close in time gather many of the same nodes while
 // those are still in the caches.  From the lane-layout connectivity cl and
 // its inverse (off, pos) alone.  Returns false, leaving the caller's order,
-// when that order keeps its locality.  The order changes which element a
-// workgroup takes, not where its residuals are stored: results are the same
-// bits either way.
+// when that order keeps its locality.  The element-node residuals are then
+// stored at their traversal position, so that the node update also reads
+// neighbouring elements' records from nearby memory, and summed through node
+// lists kept in element order: the results are the same bits either way.
 static bool locality_order(const std::vector<int>& cl, const std::vector<int>& off, const std::vector<int>& pos,
                            int nelem, int npoin, std::vector<int>& order) {
     if (nelem < 2) return false;
@@ -1077,6 +1090,17 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
         SOD_HIP(hipMalloc(&P.relem, sizeof(double) * (nl * NRES + 2)));
         SOD_HIP(hipMemcpy(P.inv_off, off.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice));
         if (nl > 0) SOD_HIP(hipMemcpy(P.inv_pos, pos.data(), sizeof(int) * nl, hipMemcpyHostToDevice));
+        // with a traversal order the stage kernel stores the element at
+        // position r at r*64 + lane: the node lists it is summed through keep
+        // their element order, so the sums are the same bits
+        P.inv_pos_r = P.inv_pos;
+        if (P.eorder != nullptr) {
+            std::vector<int> rank(ne);
+            for (size_t r = 0; r < ne; ++r) rank[(size_t)order[r]] = (int)r;
+            for (size_t q = 0; q < nl; ++q) pos[q] = rank[(size_t)(pos[q] >> 6)] * 64 + (pos[q] & 63);
+            SOD_HIP(hipMalloc(&P.inv_pos_r, sizeof(int) * (nl + 1)));
+            if (nl > 0) SOD_HIP(hipMemcpy(P.inv_pos_r, pos.data(), sizeof(int) * nl, hipMemcpyHostToDevice));
+        }
         // the same lists in lpoin_w order
         std::vector<int> woff((size_t)npoin_w + 1, 0);
         bool increasing = true;
@@ -1087,10 +1111,10 @@ static int prepare(Prepared& P, int nelem, int npoin, int npoin_w, const int* co
         }
         SOD_HIP(hipMalloc(&P.woff, sizeof(int) * ((size_t)npoin_w + 1)));
         if (increasing && (size_t)woff[npoin_w] == nl) {
-            // the working nodes' lists follow each other in inv_pos (nodes
+            // the working nodes' lists follow each other in inv_pos_r (nodes
             // outside lpoin_w hold no positions: connec never names them)
             for (int t = 0; t <= npoin_w; ++t) woff[t] += npoin_w > 0 ? off[hw[0] - 1] : 0;
-            P.wpos = P.inv_pos;
+            P.wpos = P.inv_pos_r;
         } else {
             std::vector<int> wp((size_t)woff[npoin_w] + 1);
             for (int t = 0; t < npoin_w; ++t) {
@@ -1330,7 +1354,7 @@ static void launch_rhs_t(hipStream_t s, int nelem, const Prepared& P, const doub
                          const StageParams& sp) {
     if constexpr (NW == 0) {
         hipLaunchKernelGGL((k_stage_rhs<HS, 1, ASM_PULL, TH, true>), dim3(persist_grid(nelem)), dim3(64), 0, s,
-                           nelem, (const int*)nullptr, P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2],
+                           nelem, (const int*)P.eorder, P.conn, P.He, P.gpv, P.mue, P.musgs, in[0], in[1], in[2],
                            in[3], in[4], in[5], P.relem, np, sp);
     } else if constexpr (AS == ASM_COLOR) {
         const int nc = (int)P.color_off.size() - 1;
@@ -1434,7 +1458,7 @@ static StepState step_state(Prepared& P, const FusedArgs& a) {
     st.partial = P.work + 11 * np;
     st.relem = P.relem;
     st.inv_off = P.inv_off;
-    st.inv_pos = P.inv_pos;
+    st.inv_pos = P.inv_pos_r;
     st.woff = P.woff;
     st.wpos = P.wpos;
     st.Rn = P.rn;
@@ -1661,7 +1685,7 @@ int full_rhs(int nelem, int npoin, int npoin_w, const int* connec, const int* lp
     }
     if (npoin > 0)
         hipLaunchKernelGGL(k_assemble_nodes, dim3(std::max(1, grid_for(npoin, 256))), dim3(256), 0, s, npoin,
-                           Pm.inv_off, Pm.inv_pos, Pm.relem, Rmass, Rmom, Rener, Reta);
+                           Pm.inv_off, Pm.inv_pos_r, Pm.relem, Rmass, Rmom, Rener, Reta);
     SOD_HIP(hipGetLastError());
     return SOD_OK;
 }
