@@ -304,9 +304,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3, 8)))
 #pragma unroll
         for (int f = 0; f < NRES; ++f) relem[f * rstride + n] += res[f];
     } else {
-        // The residuals of this element at this node, stored in element-node
-        // order; k_stage_update and k_assemble_nodes sum each node's
-        // contributions (pull assembly: no atomics, a fixed summation order).
+        // The residuals of this element at this node, stored at its
+        // traversal position's element-node slot; k_stage_update and
+        // k_assemble_nodes sum each node's contributions (pull assembly: no
+        // atomics, a fixed summation order).
         double2* o = reinterpret_cast<double2*>(relem + ((size_t)tp * 64 + l) * NRES);
         o[0] = make_double2(res[0], res[1]);
         o[1] = make_double2(res[2], res[3]);
